@@ -1,0 +1,164 @@
+/*
+ * ptd.h -- C ABI of libptd, the MI355X (gfx950) pseudo-transient diffusion library.
+ *
+ * The reference (PTsolvers/PseudoTransientDiffusion.jl) has no native library: each script defines its
+ * kernels as ParallelStencil functions and launches them with `@parallel f(args...)`, then runs the
+ * pseudo-transient (PT) loop on the host. This header is the boundary that replaces those launches and
+ * that loop. A Julia host binds it with `ccall`; the Python package in this repository binds it with
+ * ctypes. Signatures use plain C types only.
+ *
+ * Conventions
+ *  - Arrays cross the boundary as host `double*` in the reference's layout: column-major (x fastest)
+ *    with the reference's staggered shapes (scripts/diff_3D/diff_3D_nonlin_multixpu.jl:77-83):
+ *        H, Hold, D : (nx, ny, nz)
+ *        qx         : (nx-1, ny-2, nz-2)   qy : (nx-2, ny-1, nz-2)   qz : (nx-2, ny-2, nz-1)
+ *        ResH, dtau : (nx-2, ny-2, nz-2)
+ *    In 1D ny = nz = 1 and in 2D nz = 1; an absent axis contributes a factor 1 (not n-2) to every shape.
+ *  - The library owns the device copies inside an opaque context (one HIP stream per context).
+ *    Every call is synchronous from the caller's view except where noted. A context is not
+ *    thread-safe; separate contexts are independent.
+ *  - Every function returns a ptd_status; ptd_last_error() describes the most recent failure of the
+ *    calling thread.
+ */
+#ifndef PTD_H
+#define PTD_H
+
+#include <stdint.h>
+
+#ifdef __cplusplus
+extern "C" {
+#endif
+
+#define PTD_ABI_VERSION 1
+
+typedef enum ptd_status {
+    PTD_OK = 0,
+    PTD_ERR_ARG = 1,      /* invalid argument (shape, enum, NULL pointer, size mismatch) */
+    PTD_ERR_HIP = 2,      /* HIP runtime error, message in ptd_last_error() */
+    PTD_ERR_NAN = 3,      /* the residual norm became NaN: the reference's error("NaN") */
+    PTD_ERR_NOMEM = 4,    /* device allocation failed */
+    PTD_ERR_STATE = 5,    /* call not valid for this context (e.g. field absent for this case) */
+    PTD_ERR_NODEV = 6     /* no usable gfx950 device */
+} ptd_status;
+
+/* Diffusion coefficient: D = d0 (scalar), a step field D (nx,ny,nz), or D = H^3. */
+typedef enum ptd_dcase { PTD_LIN = 0, PTD_STEP = 1, PTD_NONLIN = 2 } ptd_dcase;
+
+/* Expression form of the kernels.
+ *  PTD_PLAIN: the converging scripts (diff_*_{lin,linstep,nonlin}.jl): /dx, /dt, and the residual
+ *             flux qH2 taken from the H *before* the last update (scripts/diff_3D/diff_3D_nonlin_multixpu.jl:41-43).
+ *  PTD_PERF : the perf scripts (diff_*_nonlin_perf.jl): multiplications by _dx = 1/dx, _dt = 1/dt and
+ *             the residual flux recomputed from the current H (scripts/diff_3D/diff_3D_nonlin_perf.jl:44-49). */
+typedef enum ptd_form { PTD_PLAIN = 0, PTD_PERF = 1 } ptd_form;
+
+/* Arithmetic of the PT sweeps.
+ *  PTD_FP64 : as the reference (@init_parallel_stencil(..., Float64, ...)).
+ *  PTD_FP32 : fields and sweeps in float32.
+ *  PTD_MIXED: float32 sweeps inside a float64 defect-correction loop: the residual of the float64
+ *             backward-Euler system and the accumulated solution stay float64 on the device. */
+typedef enum ptd_precision { PTD_FP64 = 0, PTD_FP32 = 1, PTD_MIXED = 2 } ptd_precision;
+
+/* Norm that stops the PT loop. The reference stops on RMS = norm(ResH)/sqrt(length(ResH))
+ * (scripts/diff_3D/diff_3D_nonlin_perf.jl:106); PTD_NORM_MAX stops on max|ResH| instead. Both are
+ * always computed. */
+typedef enum ptd_norm { PTD_NORM_RMS = 0, PTD_NORM_MAX = 1 } ptd_norm;
+
+typedef enum ptd_field {
+    PTD_F_H = 0, PTD_F_HOLD = 1, PTD_F_D = 2,
+    PTD_F_QX = 3, PTD_F_QY = 4, PTD_F_QZ = 5,
+    PTD_F_QX2 = 6, PTD_F_QY2 = 7, PTD_F_QZ2 = 8,
+    PTD_F_RESH = 9, PTD_F_DTAU = 10
+} ptd_field;
+
+/* Problem description. The scalar block carries the values the scripts compute in their
+ * "Derived numerics" section (e.g. scripts/diff_3D/diff_3D_nonlin_perf.jl:73-78,
+ * scripts/diff_3D/diff_3D_lin_multixpu.jl:58-62); the library uses them as given, exactly as the
+ * reference kernels receive them as arguments. */
+typedef struct ptd_params {
+    int32_t ndim;       /* 1, 2 or 3 */
+    int32_t dcase;      /* ptd_dcase */
+    int32_t form;       /* ptd_form */
+    int32_t precision;  /* ptd_precision */
+    int64_t nx, ny, nz; /* cells per axis, >= 3 on every active axis; 1 on absent axes */
+    int64_t itmax;      /* PT iteration cap per physical step (itMax) */
+    int32_t nout;       /* residual check interval (nout) */
+    int32_t stop_norm;  /* ptd_norm */
+    double dx, dy, dz;  /* cell sizes */
+    double dt;          /* physical time step (theta_kin for the reaction scripts) */
+    double vpdt;        /* Vpdtau = CFL*min(dx,dy,dz) */
+    double max_l;       /* max(lx,ly,lz) (lx in 1D) */
+    double max_l2;      /* max_l^2 */
+    double resc;        /* Resc (1/1.2 for 2D/3D nonlin, 1 otherwise) */
+    double d0;          /* lin: scalar D */
+    double re;          /* lin/step: Re */
+    double theta_r;     /* lin/step: theta_r_dtau */
+    double dtau_rho;    /* lin: scalar dtau_rho */
+    double tol;         /* PT tolerance on the stopping norm */
+    double norm_len;    /* err = ||ResH|| / sqrt(norm_len); 0 selects length(ResH) */
+} ptd_params;
+
+typedef struct ptd_ctx ptd_ctx;
+
+/* ---- library / context --------------------------------------------------------------------- */
+
+int32_t     ptd_abi_version(void);
+const char* ptd_last_error(void);
+/* Number of visible devices and whether device `dev` is gfx950 (returns PTD_ERR_NODEV otherwise). */
+int32_t     ptd_device_check(int32_t dev, char* name_out, int32_t name_len);
+
+/* Allocates every device field for `params` (H and q are double-buffered) and zeroes them, like
+ * the scripts' @zeros allocations (scripts/diff_3D/diff_3D_nonlin_multixpu.jl:77-83). */
+int32_t ptd_ctx_create(const ptd_params* params, int32_t device, ptd_ctx** out);
+int32_t ptd_ctx_destroy(ptd_ctx* ctx);
+/* Replaces the scalar block (e.g. to run the reaction scripts' theta_kin loop); shapes must match. */
+int32_t ptd_ctx_set_params(ptd_ctx* ctx, const ptd_params* params);
+/* Device bytes held by the context. */
+int64_t ptd_ctx_device_bytes(const ptd_ctx* ctx);
+/* The context's HIP stream (hipStream_t), for callers that time or order work on it. */
+void*   ptd_ctx_stream(ptd_ctx* ctx);
+
+/* Host <-> device copies of one field in the reference layout (count = number of doubles). Setting H
+ * writes both H buffers, so the Dirichlet boundary cells hold the given values in either one. */
+int32_t ptd_set_field(ptd_ctx* ctx, int32_t field, const double* host, int64_t count);
+int32_t ptd_get_field(ptd_ctx* ctx, int32_t field, double* host, int64_t count);
+/* Number of doubles of a field for this context's shape (0 if absent for the case). */
+int64_t ptd_field_count(const ptd_ctx* ctx, int32_t field);
+
+/* ---- kernel-level entry points: one reference kernel launch each --------------------------- */
+
+/* compute_iter_params!: dtau_rho = Vpdtau*max_l/@maxloc(D)/Re (step case)
+ * (scripts/diff_3D/diff_3D_linstep_multixpu.jl:24-27, diff_2D_linstep.jl:18-21, diff_1D_linstep.jl:17-20). */
+int32_t ptd_compute_iter_params(ptd_ctx* ctx);
+/* compute_flux!: damped fluxes qx, qy, qz from H; the plain form also writes qx2, qy2, qz2
+ * (scripts/diff_3D/diff_3D_nonlin_multixpu.jl:37-45, diff_3D_nonlin_perf.jl:32-37, and the 1D/2D
+ * scripts' compute_flux!). */
+int32_t ptd_compute_flux(ptd_ctx* ctx);
+/* compute_update!: H update on the inner cells (scripts/diff_3D/diff_3D_nonlin_multixpu.jl:47-50,
+ * diff_3D_nonlin_perf.jl:39-42). */
+int32_t ptd_compute_update(ptd_ctx* ctx);
+/* compute_flux_res!: qx2, qy2, qz2 from the current H (scripts/diff_3D/diff_3D_nonlin_perf.jl:44-49). */
+int32_t ptd_compute_flux_res(ptd_ctx* ctx);
+/* check_res! and the error: writes ResH and returns err_rms = norm(ResH)/sqrt(norm_len) and
+ * err_max = max|ResH| (scripts/diff_3D/diff_3D_nonlin_multixpu.jl:52-55,113; diff_2D_nonlin_perf.jl:44-47). */
+int32_t ptd_check_res(ptd_ctx* ctx, double* err_rms, double* err_max);
+/* assign! / Hold .= H (scripts/diff_3D/diff_3D_nonlin_perf.jl:56-59,110). */
+int32_t ptd_assign(ptd_ctx* ctx);
+
+/* ---- loop-level entry points: the PT loop, device resident --------------------------------- */
+
+/* `niter` PT iterations (flux + update, fused into one launch each) with no residual check: the
+ * perf scripts' inner loop body (scripts/diff_3D/diff_3D_nonlin_perf.jl:100-102). */
+int32_t ptd_iterate(ptd_ctx* ctx, int64_t niter);
+/* Same as ptd_iterate, bracketed by HIP events on the context's stream; *ms = elapsed time. */
+int32_t ptd_iterate_timed(ptd_ctx* ctx, int64_t niter, double* ms);
+/* One PT solve at fixed Hold: `while err > tol && iter < itmax` with a check every nout iterations
+ * (scripts/diff_3D/diff_3D_nonlin_multixpu.jl:102-115). Returns PTD_ERR_NAN if the error is NaN. */
+int32_t ptd_pt_solve(ptd_ctx* ctx, int64_t* iters, double* err_rms, double* err_max);
+/* The physical-time loop: nt x { ptd_pt_solve; Hold .= H } (scripts/diff_3D/diff_3D_nonlin_multixpu.jl:101-119).
+ * iters[nt] and errs[nt] (may be NULL) receive per-step iteration counts and final errors. */
+int32_t ptd_solve(ptd_ctx* ctx, int32_t nt, int64_t* iters, double* errs);
+
+#ifdef __cplusplus
+}
+#endif
+#endif /* PTD_H */

@@ -1,0 +1,466 @@
+// ptd_api.hip -- the C ABI of libptd (include/ptd.h): context and field management, the
+// kernel-level entry points (one reference kernel each) and the device-resident PT loop.
+#include <hip/hip_runtime.h>
+
+#include <cmath>
+#include <cstdio>
+#include <cstring>
+#include <string>
+#include <vector>
+
+#include "ptd_internal.hpp"
+
+namespace ptd {
+int64_t partial_capacity(const ptd_params& p, int64_t nR);
+}
+
+namespace {
+
+thread_local std::string g_err;
+
+int fail(int code, const std::string& msg) {
+    g_err = msg;
+    return code;
+}
+
+#define PTD_HIP(expr)                                                                          \
+    do {                                                                                       \
+        hipError_t e_ = (expr);                                                                \
+        if (e_ != hipSuccess) return fail(PTD_ERR_HIP, std::string(#expr) + ": " + hipGetErrorString(e_)); \
+    } while (0)
+
+#define PTD_REQUIRE(cond, code, msg) \
+    do {                             \
+        if (!(cond)) return fail((code), (msg)); \
+    } while (0)
+
+int check_params(const ptd_params* p) {
+    PTD_REQUIRE(p != nullptr, PTD_ERR_ARG, "params is NULL");
+    PTD_REQUIRE(p->ndim >= 1 && p->ndim <= 3, PTD_ERR_ARG, "ndim must be 1, 2 or 3");
+    PTD_REQUIRE(p->dcase >= PTD_LIN && p->dcase <= PTD_NONLIN, PTD_ERR_ARG, "dcase must be PTD_LIN, PTD_STEP or PTD_NONLIN");
+    PTD_REQUIRE(p->form == PTD_PLAIN || p->form == PTD_PERF, PTD_ERR_ARG, "form must be PTD_PLAIN or PTD_PERF");
+    PTD_REQUIRE(p->precision == PTD_FP64 || p->precision == PTD_FP32, PTD_ERR_ARG,
+                "precision must be PTD_FP64 or PTD_FP32 for a context");
+    PTD_REQUIRE(p->stop_norm == PTD_NORM_RMS || p->stop_norm == PTD_NORM_MAX, PTD_ERR_ARG, "stop_norm must be PTD_NORM_RMS or PTD_NORM_MAX");
+    PTD_REQUIRE(p->nx >= 3, PTD_ERR_ARG, "nx must be >= 3");
+    PTD_REQUIRE(p->ndim < 2 || p->ny >= 3, PTD_ERR_ARG, "ny must be >= 3");
+    PTD_REQUIRE(p->ndim < 3 || p->nz >= 3, PTD_ERR_ARG, "nz must be >= 3");
+    PTD_REQUIRE(p->nout >= 1, PTD_ERR_ARG, "nout must be >= 1");
+    PTD_REQUIRE(p->itmax >= 0, PTD_ERR_ARG, "itmax must be >= 0");
+    PTD_REQUIRE(p->dx > 0 && (p->ndim < 2 || p->dy > 0) && (p->ndim < 3 || p->dz > 0), PTD_ERR_ARG, "cell sizes must be > 0");
+    PTD_REQUIRE(p->dt > 0 && p->vpdt > 0 && p->max_l > 0, PTD_ERR_ARG, "dt, vpdt and max_l must be > 0");
+    const int64_t ny = p->ndim >= 2 ? p->ny : 1, nz = p->ndim >= 3 ? p->nz : 1;
+    const double cells = (double)p->nx * (double)ny * (double)nz;
+    PTD_REQUIRE(cells < 9.0e18 / 8.0, PTD_ERR_ARG, "grid too large");
+    return PTD_OK;
+}
+
+void shapes(const ptd_params& p, int64_t& nH, int64_t nQ[3], int64_t& nR) {
+    const int64_t nx = p.nx, ny = p.ndim >= 2 ? p.ny : 1, nz = p.ndim >= 3 ? p.nz : 1;
+    const int64_t mx = nx - 2, my = p.ndim >= 2 ? ny - 2 : 1, mz = p.ndim >= 3 ? nz - 2 : 1;
+    nH = nx * ny * nz;
+    nQ[0] = (nx - 1) * my * mz;
+    nQ[1] = p.ndim >= 2 ? mx * (ny - 1) * mz : 0;
+    nQ[2] = p.ndim >= 3 ? mx * my * (nz - 1) : 0;
+    nR = mx * my * mz;
+}
+
+void* field_ptr(ptd_ctx* c, int field, int64_t* count) {
+    switch (field) {
+        case PTD_F_H: *count = c->nH; return c->H[c->hcur];
+        case PTD_F_HOLD: *count = c->nH; return c->Hold;
+        case PTD_F_D: *count = c->D ? c->nH : 0; return c->D;
+        case PTD_F_QX: case PTD_F_QY: case PTD_F_QZ: {
+            const int a = field - PTD_F_QX;
+            *count = c->nQ[a];
+            return c->q[a][c->qcur];
+        }
+        case PTD_F_QX2: case PTD_F_QY2: case PTD_F_QZ2: {
+            const int a = field - PTD_F_QX2;
+            *count = c->nQ[a];
+            return c->q2[a];
+        }
+        case PTD_F_RESH: *count = c->nR; return c->resh;
+        case PTD_F_DTAU: *count = c->dtau ? c->nR : 0; return c->dtau;
+        default: *count = -1; return nullptr;
+    }
+}
+
+int dev_alloc(ptd_ctx* c, void** ptr, int64_t n, size_t esz) {
+    *ptr = nullptr;
+    if (n <= 0) return PTD_OK;
+    const size_t bytes = (size_t)n * esz;
+    hipError_t e = hipMalloc(ptr, bytes);
+    if (e != hipSuccess) {
+        *ptr = nullptr;
+        return fail(PTD_ERR_NOMEM, "hipMalloc of " + std::to_string(bytes) + " bytes failed: " + hipGetErrorString(e));
+    }
+    e = hipMemsetAsync(*ptr, 0, bytes, c->stream);
+    if (e != hipSuccess) return fail(PTD_ERR_HIP, std::string("hipMemsetAsync: ") + hipGetErrorString(e));
+    c->bytes += (int64_t)bytes;
+    return PTD_OK;
+}
+
+void free_ctx(ptd_ctx* c) {
+    if (!c) return;
+    (void)hipSetDevice(c->device);
+    if (c->stream) (void)hipStreamSynchronize(c->stream);
+    for (int b = 0; b < 2; ++b) {
+        (void)hipFree(c->H[b]);
+        for (int a = 0; a < 3; ++a) (void)hipFree(c->q[a][b]);
+    }
+    for (int a = 0; a < 3; ++a) (void)hipFree(c->q2[a]);
+    (void)hipFree(c->Hold);
+    (void)hipFree(c->D);
+    (void)hipFree(c->dtau);
+    (void)hipFree(c->resh);
+    (void)hipFree(c->partial);
+    (void)hipFree(c->dred);
+    if (c->hred) (void)hipHostFree(c->hred);
+    if (c->ev0) (void)hipEventDestroy(c->ev0);
+    if (c->ev1) (void)hipEventDestroy(c->ev1);
+    if (c->stream) (void)hipStreamDestroy(c->stream);
+    delete c;
+}
+
+// Copies between a host double array and a device array of the sweep precision.
+int upload(ptd_ctx* c, void* dst, const double* src, int64_t n) {
+    if (c->esz == 8) {
+        PTD_HIP(hipMemcpyAsync(dst, src, (size_t)n * 8, hipMemcpyHostToDevice, c->stream));
+        PTD_HIP(hipStreamSynchronize(c->stream));
+        return PTD_OK;
+    }
+    const int64_t chunk = 1 << 24;
+    std::vector<float> tmp((size_t)std::min<int64_t>(n, chunk));
+    for (int64_t o = 0; o < n; o += chunk) {
+        const int64_t m = std::min<int64_t>(chunk, n - o);
+        for (int64_t t = 0; t < m; ++t) tmp[(size_t)t] = (float)src[o + t];
+        PTD_HIP(hipMemcpyAsync((float*)dst + o, tmp.data(), (size_t)m * 4, hipMemcpyHostToDevice, c->stream));
+        PTD_HIP(hipStreamSynchronize(c->stream));
+    }
+    return PTD_OK;
+}
+
+int download(ptd_ctx* c, double* dst, const void* src, int64_t n) {
+    if (c->esz == 8) {
+        PTD_HIP(hipMemcpyAsync(dst, src, (size_t)n * 8, hipMemcpyDeviceToHost, c->stream));
+        PTD_HIP(hipStreamSynchronize(c->stream));
+        return PTD_OK;
+    }
+    const int64_t chunk = 1 << 24;
+    std::vector<float> tmp((size_t)std::min<int64_t>(n, chunk));
+    for (int64_t o = 0; o < n; o += chunk) {
+        const int64_t m = std::min<int64_t>(chunk, n - o);
+        PTD_HIP(hipMemcpyAsync(tmp.data(), (const float*)src + o, (size_t)m * 4, hipMemcpyDeviceToHost, c->stream));
+        PTD_HIP(hipStreamSynchronize(c->stream));
+        for (int64_t t = 0; t < m; ++t) dst[o + t] = (double)tmp[(size_t)t];
+    }
+    return PTD_OK;
+}
+
+int device_guard(ptd_ctx* c) {
+    PTD_REQUIRE(c != nullptr, PTD_ERR_ARG, "ctx is NULL");
+    PTD_HIP(hipSetDevice(c->device));
+    return PTD_OK;
+}
+
+// Reads (sum r^2, max|r|) of the last reduction and forms the two norms.
+int fetch_errors(ptd_ctx* c, double* rms, double* mx) {
+    PTD_HIP(hipMemcpyAsync(c->hred, c->dred, 2 * sizeof(double), hipMemcpyDeviceToHost, c->stream));
+    PTD_HIP(hipStreamSynchronize(c->stream));
+    const double len = c->p.norm_len > 0 ? c->p.norm_len : (double)c->nR;
+    *rms = std::sqrt(c->hred[0]) / std::sqrt(len);
+    *mx = c->hred[1];
+    return PTD_OK;
+}
+
+// One PT iteration; on check iterations the residual reduction is left in dred.
+int one_iteration(ptd_ctx* c, bool check) {
+    const bool plain = c->p.form == PTD_PLAIN;
+    if (ptd::fused_available(c)) {
+        PTD_HIP(ptd::launch_fused(c, check && plain));
+        if (check && !plain) {
+            PTD_HIP(ptd::launch_flux(c, 2));
+            PTD_HIP(ptd::launch_residual(c, false));
+        }
+        return PTD_OK;
+    }
+    // reference structure: compute_flux! (plain: residual fluxes from the same, pre-update H), compute_update!
+    PTD_HIP(ptd::launch_flux(c, plain && check ? 1 : 0));
+    PTD_HIP(ptd::launch_update(c));
+    if (check) {
+        if (!plain) PTD_HIP(ptd::launch_flux(c, 2));
+        PTD_HIP(ptd::launch_residual(c, false));
+    }
+    return PTD_OK;
+}
+
+}  // namespace
+
+extern "C" {
+
+int32_t ptd_abi_version(void) { return PTD_ABI_VERSION; }
+
+const char* ptd_last_error(void) { return g_err.c_str(); }
+
+int32_t ptd_device_check(int32_t dev, char* name_out, int32_t name_len) {
+    int n = 0;
+    hipError_t e = hipGetDeviceCount(&n);
+    if (e != hipSuccess || n <= 0) return fail(PTD_ERR_NODEV, "no HIP device visible");
+    if (dev < 0 || dev >= n) return fail(PTD_ERR_NODEV, "device index out of range");
+    hipDeviceProp_t prop;
+    PTD_HIP(hipGetDeviceProperties(&prop, dev));
+    if (name_out && name_len > 0) {
+        std::snprintf(name_out, (size_t)name_len, "%s", prop.gcnArchName);
+    }
+    if (std::strncmp(prop.gcnArchName, "gfx950", 6) != 0)
+        return fail(PTD_ERR_NODEV, std::string("device is ") + prop.gcnArchName + ", libptd is built for gfx950");
+    return PTD_OK;
+}
+
+int32_t ptd_ctx_create(const ptd_params* params, int32_t device, ptd_ctx** out) {
+    PTD_REQUIRE(out != nullptr, PTD_ERR_ARG, "out is NULL");
+    *out = nullptr;
+    int st = check_params(params);
+    if (st != PTD_OK) return st;
+    st = ptd_device_check(device, nullptr, 0);
+    if (st != PTD_OK) return st;
+    PTD_HIP(hipSetDevice(device));
+    ptd_ctx* c = new ptd_ctx();
+    std::memset((void*)c, 0, sizeof(*c));
+    c->p = *params;
+    if (c->p.ndim < 2) c->p.ny = 1;
+    if (c->p.ndim < 3) c->p.nz = 1;
+    c->device = device;
+    c->esz = params->precision == PTD_FP32 ? 4 : 8;
+    shapes(c->p, c->nH, c->nQ, c->nR);
+    hipError_t e = hipStreamCreate(&c->stream);
+    if (e != hipSuccess) {
+        free_ctx(c);
+        return fail(PTD_ERR_HIP, std::string("hipStreamCreate: ") + hipGetErrorString(e));
+    }
+#define PTD_ALLOC(ptr, n, esz)                 \
+    do {                                       \
+        int s_ = dev_alloc(c, (ptr), (n), (esz)); \
+        if (s_ != PTD_OK) {                    \
+            free_ctx(c);                       \
+            return s_;                         \
+        }                                      \
+    } while (0)
+    for (int b = 0; b < 2; ++b) {
+        PTD_ALLOC(&c->H[b], c->nH, c->esz);
+        for (int a = 0; a < 3; ++a) PTD_ALLOC(&c->q[a][b], c->nQ[a], c->esz);
+    }
+    for (int a = 0; a < 3; ++a) PTD_ALLOC(&c->q2[a], c->nQ[a], c->esz);
+    PTD_ALLOC(&c->Hold, c->nH, c->esz);
+    if (c->p.dcase == PTD_STEP) {
+        PTD_ALLOC(&c->D, c->nH, c->esz);
+        PTD_ALLOC(&c->dtau, c->nR, c->esz);
+    }
+    PTD_ALLOC(&c->resh, c->nR, c->esz);
+    c->npartial = ptd::partial_capacity(c->p, c->nR);
+    PTD_ALLOC((void**)&c->partial, 2 * c->npartial, sizeof(double));
+    PTD_ALLOC((void**)&c->dred, 2, sizeof(double));
+#undef PTD_ALLOC
+    e = hipHostMalloc((void**)&c->hred, 2 * sizeof(double), hipHostMallocDefault);
+    if (e == hipSuccess) e = hipEventCreate(&c->ev0);
+    if (e == hipSuccess) e = hipEventCreate(&c->ev1);
+    if (e == hipSuccess) e = hipStreamSynchronize(c->stream);
+    if (e != hipSuccess) {
+        free_ctx(c);
+        return fail(PTD_ERR_HIP, std::string("context setup: ") + hipGetErrorString(e));
+    }
+    *out = c;
+    return PTD_OK;
+}
+
+int32_t ptd_ctx_destroy(ptd_ctx* ctx) {
+    free_ctx(ctx);
+    return PTD_OK;
+}
+
+int32_t ptd_ctx_set_params(ptd_ctx* ctx, const ptd_params* params) {
+    int st = device_guard(ctx);
+    if (st != PTD_OK) return st;
+    st = check_params(params);
+    if (st != PTD_OK) return st;
+    const ptd_params& o = ctx->p;
+    PTD_REQUIRE(params->ndim == o.ndim && params->nx == o.nx && (o.ndim < 2 || params->ny == o.ny) &&
+                    (o.ndim < 3 || params->nz == o.nz) && params->dcase == o.dcase &&
+                    params->precision == o.precision,
+                PTD_ERR_ARG, "set_params may not change shape, dcase or precision");
+    ptd_params np = *params;
+    np.ny = o.ny;
+    np.nz = o.nz;
+    ctx->p = np;
+    return PTD_OK;
+}
+
+int64_t ptd_ctx_device_bytes(const ptd_ctx* ctx) { return ctx ? ctx->bytes : 0; }
+
+void* ptd_ctx_stream(ptd_ctx* ctx) { return ctx ? (void*)ctx->stream : nullptr; }
+
+int64_t ptd_field_count(const ptd_ctx* ctx, int32_t field) {
+    if (!ctx) return -1;
+    int64_t n = -1;
+    field_ptr(const_cast<ptd_ctx*>(ctx), field, &n);
+    return n;
+}
+
+int32_t ptd_set_field(ptd_ctx* ctx, int32_t field, const double* host, int64_t count) {
+    int st = device_guard(ctx);
+    if (st != PTD_OK) return st;
+    int64_t n = 0;
+    void* dst = field_ptr(ctx, field, &n);
+    PTD_REQUIRE(n >= 0, PTD_ERR_ARG, "unknown field");
+    PTD_REQUIRE(n > 0 && dst != nullptr, PTD_ERR_STATE, "field is absent for this case");
+    PTD_REQUIRE(host != nullptr, PTD_ERR_ARG, "host pointer is NULL");
+    PTD_REQUIRE(count == n, PTD_ERR_ARG, "count " + std::to_string(count) + " != field size " + std::to_string(n));
+    st = upload(ctx, dst, host, n);
+    if (st != PTD_OK) return st;
+    if (field == PTD_F_H) {
+        PTD_HIP(hipMemcpyAsync(ctx->H[1 - ctx->hcur], dst, (size_t)n * ctx->esz, hipMemcpyDeviceToDevice, ctx->stream));
+        PTD_HIP(hipStreamSynchronize(ctx->stream));
+    }
+    return PTD_OK;
+}
+
+int32_t ptd_get_field(ptd_ctx* ctx, int32_t field, double* host, int64_t count) {
+    int st = device_guard(ctx);
+    if (st != PTD_OK) return st;
+    int64_t n = 0;
+    void* src = field_ptr(ctx, field, &n);
+    PTD_REQUIRE(n >= 0, PTD_ERR_ARG, "unknown field");
+    PTD_REQUIRE(n > 0 && src != nullptr, PTD_ERR_STATE, "field is absent for this case");
+    PTD_REQUIRE(host != nullptr, PTD_ERR_ARG, "host pointer is NULL");
+    PTD_REQUIRE(count == n, PTD_ERR_ARG, "count " + std::to_string(count) + " != field size " + std::to_string(n));
+    return download(ctx, host, src, n);
+}
+
+int32_t ptd_compute_iter_params(ptd_ctx* ctx) {
+    int st = device_guard(ctx);
+    if (st != PTD_OK) return st;
+    PTD_REQUIRE(ctx->p.dcase == PTD_STEP, PTD_ERR_STATE, "compute_iter_params needs the step case");
+    PTD_HIP(ptd::launch_iter_params(ctx));
+    PTD_HIP(hipStreamSynchronize(ctx->stream));
+    return PTD_OK;
+}
+
+int32_t ptd_compute_flux(ptd_ctx* ctx) {
+    int st = device_guard(ctx);
+    if (st != PTD_OK) return st;
+    PTD_HIP(ptd::launch_flux(ctx, ctx->p.form == PTD_PLAIN ? 1 : 0));
+    PTD_HIP(hipStreamSynchronize(ctx->stream));
+    return PTD_OK;
+}
+
+int32_t ptd_compute_update(ptd_ctx* ctx) {
+    int st = device_guard(ctx);
+    if (st != PTD_OK) return st;
+    PTD_HIP(ptd::launch_update(ctx));
+    PTD_HIP(hipStreamSynchronize(ctx->stream));
+    return PTD_OK;
+}
+
+int32_t ptd_compute_flux_res(ptd_ctx* ctx) {
+    int st = device_guard(ctx);
+    if (st != PTD_OK) return st;
+    PTD_HIP(ptd::launch_flux(ctx, 2));
+    PTD_HIP(hipStreamSynchronize(ctx->stream));
+    return PTD_OK;
+}
+
+int32_t ptd_check_res(ptd_ctx* ctx, double* err_rms, double* err_max) {
+    int st = device_guard(ctx);
+    if (st != PTD_OK) return st;
+    PTD_HIP(ptd::launch_residual(ctx, true));
+    double rms = 0, mx = 0;
+    st = fetch_errors(ctx, &rms, &mx);
+    if (st != PTD_OK) return st;
+    if (err_rms) *err_rms = rms;
+    if (err_max) *err_max = mx;
+    return PTD_OK;
+}
+
+int32_t ptd_assign(ptd_ctx* ctx) {
+    int st = device_guard(ctx);
+    if (st != PTD_OK) return st;
+    PTD_HIP(hipMemcpyAsync(ctx->Hold, ctx->H[ctx->hcur], (size_t)ctx->nH * ctx->esz, hipMemcpyDeviceToDevice, ctx->stream));
+    PTD_HIP(hipStreamSynchronize(ctx->stream));
+    return PTD_OK;
+}
+
+int32_t ptd_iterate(ptd_ctx* ctx, int64_t niter) {
+    int st = device_guard(ctx);
+    if (st != PTD_OK) return st;
+    PTD_REQUIRE(niter >= 0, PTD_ERR_ARG, "niter must be >= 0");
+    for (int64_t it = 0; it < niter; ++it) {
+        st = one_iteration(ctx, false);
+        if (st != PTD_OK) return st;
+    }
+    PTD_HIP(hipStreamSynchronize(ctx->stream));
+    return PTD_OK;
+}
+
+int32_t ptd_iterate_timed(ptd_ctx* ctx, int64_t niter, double* ms) {
+    int st = device_guard(ctx);
+    if (st != PTD_OK) return st;
+    PTD_REQUIRE(niter >= 0, PTD_ERR_ARG, "niter must be >= 0");
+    PTD_HIP(hipEventRecord(ctx->ev0, ctx->stream));
+    for (int64_t it = 0; it < niter; ++it) {
+        st = one_iteration(ctx, false);
+        if (st != PTD_OK) return st;
+    }
+    PTD_HIP(hipEventRecord(ctx->ev1, ctx->stream));
+    PTD_HIP(hipEventSynchronize(ctx->ev1));
+    float f = 0.0f;
+    PTD_HIP(hipEventElapsedTime(&f, ctx->ev0, ctx->ev1));
+    if (ms) *ms = (double)f;
+    return PTD_OK;
+}
+
+int32_t ptd_pt_solve(ptd_ctx* ctx, int64_t* iters, double* err_rms, double* err_max) {
+    int st = device_guard(ctx);
+    if (st != PTD_OK) return st;
+    const ptd_params& p = ctx->p;
+    int64_t iter = 0;
+    double err = 2.0 * p.tol, rms = err, mx = err;
+    // while err > tol && iter < itMax (scripts/diff_3D/diff_3D_nonlin_multixpu.jl:104-115)
+    while (err > p.tol && iter < p.itmax) {
+        const bool check = ((iter + 1) % p.nout) == 0;
+        st = one_iteration(ctx, check);
+        if (st != PTD_OK) return st;
+        iter += 1;
+        if (check) {
+            st = fetch_errors(ctx, &rms, &mx);
+            if (st != PTD_OK) return st;
+            err = p.stop_norm == PTD_NORM_MAX ? mx : rms;
+        }
+    }
+    PTD_HIP(hipStreamSynchronize(ctx->stream));
+    if (iters) *iters = iter;
+    if (err_rms) *err_rms = rms;
+    if (err_max) *err_max = mx;
+    if (std::isnan(err)) return fail(PTD_ERR_NAN, "NaN");
+    return PTD_OK;
+}
+
+int32_t ptd_solve(ptd_ctx* ctx, int32_t nt, int64_t* iters, double* errs) {
+    int st = device_guard(ctx);
+    if (st != PTD_OK) return st;
+    PTD_REQUIRE(nt >= 0, PTD_ERR_ARG, "nt must be >= 0");
+    for (int32_t it = 0; it < nt; ++it) {
+        int64_t n = 0;
+        double rms = 0, mx = 0;
+        const int s = ptd_pt_solve(ctx, &n, &rms, &mx);
+        if (iters) iters[it] = n;
+        if (errs) errs[it] = ctx->p.stop_norm == PTD_NORM_MAX ? mx : rms;
+        if (s != PTD_OK && s != PTD_ERR_NAN) return s;
+        st = ptd_assign(ctx);  // Hold .= H
+        if (st != PTD_OK) return st;
+        if (s == PTD_ERR_NAN) return s;
+    }
+    return PTD_OK;
+}
+
+}  // extern "C"

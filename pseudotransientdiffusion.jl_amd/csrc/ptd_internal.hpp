@@ -1,0 +1,209 @@
+// ptd_internal.hpp -- shared definitions of libptd (not part of the C ABI).
+//
+// Device-side restatement of the reference's per-face and per-cell expressions. Every template keeps
+// the operation order of the reference source it cites, and the library is compiled with
+// -ffp-contract=off, so the float64 path rounds exactly as the reference's Julia code does.
+#pragma once
+
+#include <hip/hip_runtime.h>
+#include <stdint.h>
+
+#include "../../include/ptd.h"
+
+namespace ptd {
+
+// Float64(pi) and pi*pi as Julia evaluates them for Irrational pi (both operands rounded first).
+constexpr double kPi = 3.141592653589793115997963468544185161590576171875;
+
+// Scalars of one problem, in the sweep precision T.
+template <typename T>
+struct KP {
+    int64_t nx, ny, nz;   // cells (1 on absent axes)
+    int64_t mx, my, mz;   // inner cells (n-2 on active axes, 1 on absent ones)
+    int32_t hy, hz;       // y / z axes present
+    T dx, dy, dz, _dx, _dy, _dz, dt, _dt;
+    T pi, pi2;            // Float64(pi), pi*pi
+    T c_lv;               // max_l / Vpdtau      (first division of max_l / Vpdtau / Re * Resc)
+    T c_vl;               // Vpdtau * max_l      (first product of Vpdtau * max_l / H3 / Re * Resc)
+    T max_l2, resc, d0, theta_r, dtau_rho, re;
+};
+
+template <typename T>
+KP<T> make_kp(const ptd_params& p) {
+    KP<T> k;
+    k.nx = p.nx;
+    k.ny = p.ndim >= 2 ? p.ny : 1;
+    k.nz = p.ndim >= 3 ? p.nz : 1;
+    k.hy = p.ndim >= 2;
+    k.hz = p.ndim >= 3;
+    k.mx = k.nx - 2;
+    k.my = k.hy ? k.ny - 2 : 1;
+    k.mz = k.hz ? k.nz - 2 : 1;
+    k.dx = (T)p.dx;
+    k.dy = (T)(k.hy ? p.dy : 1.0);
+    k.dz = (T)(k.hz ? p.dz : 1.0);
+    k._dx = (T)(1.0 / p.dx);  // _dx = 1.0/dx (scripts/diff_3D/diff_3D_nonlin_perf.jl:78)
+    k._dy = (T)(k.hy ? 1.0 / p.dy : 0.0);
+    k._dz = (T)(k.hz ? 1.0 / p.dz : 0.0);
+    k.dt = (T)p.dt;
+    k._dt = (T)(1.0 / p.dt);
+    k.pi = (T)kPi;
+    k.pi2 = (T)(kPi * kPi);
+    k.c_lv = (T)(p.max_l / p.vpdt);
+    k.c_vl = (T)(p.vpdt * p.max_l);
+    k.max_l2 = (T)p.max_l2;
+    k.resc = (T)p.resc;
+    k.d0 = (T)p.d0;
+    k.theta_r = (T)p.theta_r;
+    k.dtau_rho = (T)p.dtau_rho;
+    k.re = (T)p.re;
+    return k;
+}
+
+// ---------------------------------------------------------------------------------------------
+// physics, one face or one cell
+// ---------------------------------------------------------------------------------------------
+
+// Face coefficient between the left/lower cell a and the right/upper cell b.
+//  lin   : D                                        (scripts/diff_3D/diff_3D_lin_multixpu.jl:25)
+//  step  : @av_xi(D) = (D[a] + D[b])*0.5               (scripts/diff_3D/diff_3D_linstep_multixpu.jl:30)
+//  nonlin: plain @av_xi(H)*@av_xi(H)*@av_xi(H)         (scripts/diff_3D/diff_3D_nonlin_multixpu.jl:25)
+//          perf  0.5*(a+b) * 0.5*(a+b) * 0.5*(a+b)     (scripts/diff_3D/diff_3D_nonlin_perf.jl:20)
+template <typename T, int DC, int FORM>
+__device__ __forceinline__ T face_coef(const KP<T>& p, T Ha, T Hb, T Da, T Db) {
+    if constexpr (DC == PTD_LIN) {
+        return p.d0;
+    } else if constexpr (DC == PTD_STEP) {
+        return (Da + Db) * (T)0.5;
+    } else {
+        T s = Ha + Hb;
+        if constexpr (FORM == PTD_PERF) {
+            return (T)0.5 * s * (T)0.5 * s * (T)0.5 * s;
+        } else {
+            T av = s * (T)0.5;
+            return av * av * av;
+        }
+    }
+}
+
+// theta_r_dtau at a face: scalar for lin/step; for nonlin
+//   max_l / Vpdtau / (pi + sqrt(pi*pi + max_l2 / H3 / dt)) * Resc    (scripts/diff_3D/diff_3D_nonlin_multixpu.jl:28,32)
+//   ... max_l2 / H3 * _dt ...                                        (scripts/diff_3D/diff_3D_nonlin_perf.jl:23,27)
+template <typename T, int DC, int FORM>
+__device__ __forceinline__ T face_theta(const KP<T>& p, T coef) {
+    if constexpr (DC != PTD_NONLIN) {
+        return p.theta_r;
+    } else {
+        T re;
+        if constexpr (FORM == PTD_PERF) re = p.pi + sqrt(p.pi2 + p.max_l2 / coef * p._dt);
+        else re = p.pi + sqrt(p.pi2 + p.max_l2 / coef / p.dt);
+        return p.c_lv / re * p.resc;
+    }
+}
+
+// Damped flux (scripts/diff_3D/diff_3D_nonlin_multixpu.jl:38, diff_3D_nonlin_perf.jl:33).
+template <typename T, int FORM>
+__device__ __forceinline__ T damped_flux(T q, T theta, T coef, T dH, T h, T ih) {
+    if constexpr (FORM == PTD_PERF) return (q * theta - coef * ih * dH) / ((T)1.0 + theta);
+    else return (q * theta - coef * dH / h) / ((T)1.0 + theta);
+}
+
+// Residual flux -coef*dH/dx (plain, diff_3D_nonlin_multixpu.jl:41) or -coef*_dx*dH (perf, diff_3D_nonlin_perf.jl:45).
+template <typename T, int FORM>
+__device__ __forceinline__ T res_flux(T coef, T dH, T h, T ih) {
+    if constexpr (FORM == PTD_PERF) return -coef * ih * dH;
+    else return -coef * dH / h;
+}
+
+// Nonlinear cell dtau_rho = Vpdtau * max_l / H3 / Re * Resc with Re from the cell H3
+// (scripts/diff_3D/diff_3D_nonlin_multixpu.jl:31,35; diff_3D_nonlin_perf.jl:26,30).
+template <typename T, int FORM>
+__device__ __forceinline__ T nonlin_dtau(const KP<T>& p, T H) {
+    T h3 = H * H * H;
+    T re;
+    if constexpr (FORM == PTD_PERF) re = p.pi + sqrt(p.pi2 + p.max_l2 / h3 * p._dt);
+    else re = p.pi + sqrt(p.pi2 + p.max_l2 / h3 / p.dt);
+    return p.c_vl / h3 / re * p.resc;
+}
+
+// Step dtau_rho from the neighbourhood maximum of D (compute_iter_params!,
+// scripts/diff_3D/diff_3D_linstep_multixpu.jl:24-27: Vpdtau * max_l / @maxloc(D) / Re).
+template <typename T>
+__device__ __forceinline__ T step_dtau(const KP<T>& p, T dmax) {
+    return p.c_vl / dmax / p.re;
+}
+
+// Divergence term from the three flux differences (left to right sum, absent axes dropped).
+//  plain: dqx/dx + dqy/dy + dqz/dz       (scripts/diff_3D/diff_3D_nonlin_multixpu.jl:48)
+//  perf : _dx*dqx + _dy*dqy + _dz*dqz    (scripts/diff_3D/diff_3D_nonlin_perf.jl:40)
+template <typename T, int FORM, int NDIM>
+__device__ __forceinline__ T divergence(const KP<T>& p, T ddx, T ddy, T ddz) {
+    T s;
+    if constexpr (FORM == PTD_PERF) {
+        s = p._dx * ddx;
+        if constexpr (NDIM >= 2) s = s + p._dy * ddy;
+        if constexpr (NDIM >= 3) s = s + p._dz * ddz;
+    } else {
+        s = ddx / p.dx;
+        if constexpr (NDIM >= 2) s = s + ddy / p.dy;
+        if constexpr (NDIM >= 3) s = s + ddz / p.dz;
+    }
+    return s;
+}
+
+// H update (compute_update!): plain (H + dtau*(Hold/dt - div)) / (1.0 + dtau/dt)
+// (scripts/diff_3D/diff_3D_nonlin_multixpu.jl:48); perf (H + dtau*(_dt*Hold - div)) / (1.0 + _dt*dtau)
+// (scripts/diff_3D/diff_3D_nonlin_perf.jl:40).
+template <typename T, int FORM>
+__device__ __forceinline__ T h_update(const KP<T>& p, T H, T Hold, T dtau, T div) {
+    if constexpr (FORM == PTD_PERF) return (H + dtau * (p._dt * Hold - div)) / ((T)1.0 + p._dt * dtau);
+    else return (H + dtau * (Hold / p.dt - div)) / ((T)1.0 + dtau / p.dt);
+}
+
+// Residual (check_res!): plain -(H - Hold)/dt - div (diff_3D_nonlin_multixpu.jl:53);
+// perf -_dt*(H - Hold) - div (diff_2D_nonlin_perf.jl:45).
+template <typename T, int FORM>
+__device__ __forceinline__ T residual(const KP<T>& p, T H, T Hold, T div) {
+    if constexpr (FORM == PTD_PERF) return -p._dt * (H - Hold) - div;
+    else return -(H - Hold) / p.dt - div;
+}
+
+}  // namespace ptd
+
+// ---------------------------------------------------------------------------------------------
+// context
+// ---------------------------------------------------------------------------------------------
+
+struct ptd_ctx {
+    ptd_params p;
+    int device;
+    hipStream_t stream;
+    size_t esz;                 // bytes per sweep element (8 for FP64, 4 for FP32)
+    int64_t nH, nQ[3], nR;      // element counts (nQ[a] = 0 for absent axes)
+    void* H[2];                 // double-buffered H (fused sweeps write the other buffer)
+    void* q[3][2];              // double-buffered damped fluxes
+    int hcur, qcur;
+    void* Hold;
+    void* D;                    // step field (nullptr otherwise)
+    void* dtau;                 // step dtau_rho (nullptr otherwise)
+    void* q2[3];                // residual fluxes (kernel-level API and 1D/2D plain loop)
+    void* resh;                 // ResH (kernel-level API)
+    double* partial;            // per-block (sum, max) of the residual
+    int64_t npartial;
+    double* dred;               // device (sum, max) after the final reduction
+    double* hred;               // pinned host mirror of dred
+    hipEvent_t ev0, ev1;
+    int64_t bytes;
+};
+
+namespace ptd {
+
+// launchers (ptd_kernels.hip)
+hipError_t launch_flux(ptd_ctx* c, int q2mode);       // 0: q only, 1: q and q2, 2: q2 only
+hipError_t launch_update(ptd_ctx* c);
+hipError_t launch_iter_params(ptd_ctx* c);
+hipError_t launch_residual(ptd_ctx* c, bool write_resh);  // partial sums -> dred
+hipError_t launch_fused(ptd_ctx* c, bool check);          // one fused PT iteration (3D)
+bool fused_available(const ptd_ctx* c);
+
+}  // namespace ptd

@@ -1,0 +1,554 @@
+// ptd_kernels.hip -- gfx950 kernels of libptd.
+//
+// Two families:
+//  * reference-structured kernels, one launch per reference kernel (compute_flux!, compute_update!,
+//    compute_flux_res!, check_res!, compute_iter_params!), any ndim, one thread per face or cell;
+//  * the fused 3D PT sweep: flux and H update of one PT iteration in one launch. A block is a
+//    64 x R tile of (x, y) columns that marches along z: x neighbours come from wavefront shuffles,
+//    y neighbours from one LDS plane, z neighbours from registers carried along the march. H and the
+//    fluxes are double-buffered so every iteration reads only the previous iterate (Jacobi order, the
+//    same values the reference's two in-place kernels produce).
+#include <hip/hip_runtime.h>
+
+#include <algorithm>
+
+#include "ptd_internal.hpp"
+
+namespace ptd {
+
+static constexpr int kBlock = 256;
+
+__device__ __forceinline__ double nanmax(double a, double b) {
+    return (a != a || a > b) ? a : b;
+}
+
+template <typename T>
+__device__ __forceinline__ T load_or0(const T* __restrict__ a, int64_t idx, bool ok) {
+    return ok ? a[idx] : (T)0;
+}
+
+// Sum and NaN-propagating max over a block of `nthreads` threads (a multiple of 64), written by
+// thread 0 to *out. Fixed order: wavefront butterfly, then waves in index order.
+__device__ __forceinline__ void block_sum_max(double s, double m, int tid, int nthreads, double2* out) {
+    __shared__ double ws[16], wm[16];
+#pragma unroll
+    for (int off = 32; off > 0; off >>= 1) {
+        s += __shfl_xor(s, off);
+        m = nanmax(m, __shfl_xor(m, off));
+    }
+    const int wid = tid >> 6;
+    if ((tid & 63) == 0) {
+        ws[wid] = s;
+        wm[wid] = m;
+    }
+    __syncthreads();
+    if (tid == 0) {
+        double S = 0.0, M = 0.0;
+        for (int w = 0; w < (nthreads >> 6); ++w) {
+            S += ws[w];
+            M = nanmax(wm[w], M);
+        }
+        *out = make_double2(S, M);
+    }
+}
+
+// ---------------------------------------------------------------------------------------------
+// reference-structured kernels
+// ---------------------------------------------------------------------------------------------
+
+// compute_flux! / compute_flux_res! along one axis: thread t owns face t of the q array of that axis
+// (column-major: qx (nx-1,my,mz), qy (mx,ny-1,mz), qz (mx,my,nz-1)).
+// Q2MODE 0: damped flux only (perf compute_flux!); 1: damped and residual flux from the same H
+// (plain compute_flux!, scripts/diff_3D/diff_3D_nonlin_multixpu.jl:37-45); 2: residual flux only
+// (compute_flux_res!, scripts/diff_3D/diff_3D_nonlin_perf.jl:44-49).
+template <typename T, int DC, int FORM, int AXIS, int Q2MODE>
+__global__ void __launch_bounds__(kBlock)
+k_flux(const KP<T> p, const T* __restrict__ H, const T* __restrict__ D, T* __restrict__ q,
+       T* __restrict__ q2, int64_t count) {
+    const int64_t t = (int64_t)blockIdx.x * kBlock + threadIdx.x;
+    if (t >= count) return;
+    int64_t a, step;
+    T h, ih;
+    if constexpr (AXIS == 0) {
+        const int64_t nf = p.nx - 1, f = t % nf, r = t / nf, jj = r % p.my, kk = r / p.my;
+        a = f + p.nx * ((jj + p.hy) + p.ny * (kk + p.hz));
+        step = 1;
+        h = p.dx;
+        ih = p._dx;
+    } else if constexpr (AXIS == 1) {
+        const int64_t nf = p.ny - 1, ii = t % p.mx, r = t / p.mx, f = r % nf, kk = r / nf;
+        a = (ii + 1) + p.nx * (f + p.ny * (kk + p.hz));
+        step = p.nx;
+        h = p.dy;
+        ih = p._dy;
+    } else {
+        const int64_t ii = t % p.mx, r = t / p.mx, jj = r % p.my, f = r / p.my;
+        a = (ii + 1) + p.nx * ((jj + 1) + p.ny * f);
+        step = p.nx * p.ny;
+        h = p.dz;
+        ih = p._dz;
+    }
+    const int64_t b = a + step;
+    const T Ha = H[a], Hb = H[b];
+    T Da = 0, Db = 0;
+    if constexpr (DC == PTD_STEP) {
+        Da = D[a];
+        Db = D[b];
+    }
+    const T coef = face_coef<T, DC, FORM>(p, Ha, Hb, Da, Db);
+    const T dH = Hb - Ha;
+    if constexpr (Q2MODE != 2) {
+        const T th = face_theta<T, DC, FORM>(p, coef);
+        q[t] = damped_flux<T, FORM>(q[t], th, coef, dH, h, ih);
+    }
+    if constexpr (Q2MODE >= 1) q2[t] = res_flux<T, FORM>(coef, dH, h, ih);
+}
+
+struct CellIdx {
+    int64_t h;             // H index
+    int64_t x0, x1;        // qx faces left/right
+    int64_t y0, y1;        // qy faces
+    int64_t z0, z1;        // qz faces
+};
+
+template <typename T>
+__device__ __forceinline__ CellIdx cell_index(const KP<T>& p, int64_t t) {
+    const int64_t ii = t % p.mx, r = t / p.mx, jj = r % p.my, kk = r / p.my;
+    CellIdx c;
+    c.h = (ii + 1) + p.nx * ((jj + p.hy) + p.ny * (kk + p.hz));
+    c.x0 = ii + (p.nx - 1) * (jj + p.my * kk);
+    c.x1 = c.x0 + 1;
+    c.y0 = ii + p.mx * (jj + (p.ny - 1) * kk);
+    c.y1 = c.y0 + p.mx;
+    c.z0 = ii + p.mx * (jj + p.my * kk);
+    c.z1 = c.z0 + p.mx * p.my;
+    return c;
+}
+
+// compute_update! on @inn(H), in place (reads H only at the updated cell).
+template <typename T, int DC, int FORM, int NDIM>
+__global__ void __launch_bounds__(kBlock)
+k_update(const KP<T> p, T* __restrict__ H, const T* __restrict__ Hold, const T* __restrict__ dtau,
+         const T* __restrict__ qx, const T* __restrict__ qy, const T* __restrict__ qz, int64_t count) {
+    const int64_t t = (int64_t)blockIdx.x * kBlock + threadIdx.x;
+    if (t >= count) return;
+    const CellIdx c = cell_index(p, t);
+    const T Hc = H[c.h];
+    T dtr;
+    if constexpr (DC == PTD_LIN) dtr = p.dtau_rho;
+    else if constexpr (DC == PTD_STEP) dtr = dtau[t];
+    else dtr = nonlin_dtau<T, FORM>(p, Hc);
+    const T ddx = qx[c.x1] - qx[c.x0];
+    T ddy = 0, ddz = 0;
+    if constexpr (NDIM >= 2) ddy = qy[c.y1] - qy[c.y0];
+    if constexpr (NDIM >= 3) ddz = qz[c.z1] - qz[c.z0];
+    const T div = divergence<T, FORM, NDIM>(p, ddx, ddy, ddz);
+    H[c.h] = h_update<T, FORM>(p, Hc, Hold[c.h], dtr, div);
+}
+
+// check_res!: ResH (optional) and per-block (sum r^2, max |r|).
+template <typename T, int FORM, int NDIM>
+__global__ void __launch_bounds__(kBlock)
+k_residual(const KP<T> p, const T* __restrict__ H, const T* __restrict__ Hold, const T* __restrict__ qx2,
+           const T* __restrict__ qy2, const T* __restrict__ qz2, T* __restrict__ resh,
+           double2* __restrict__ partial, int64_t count) {
+    const int64_t t = (int64_t)blockIdx.x * kBlock + threadIdx.x;
+    double s = 0.0, m = 0.0;
+    if (t < count) {
+        const CellIdx c = cell_index(p, t);
+        const T ddx = qx2[c.x1] - qx2[c.x0];
+        T ddy = 0, ddz = 0;
+        if constexpr (NDIM >= 2) ddy = qy2[c.y1] - qy2[c.y0];
+        if constexpr (NDIM >= 3) ddz = qz2[c.z1] - qz2[c.z0];
+        const T div = divergence<T, FORM, NDIM>(p, ddx, ddy, ddz);
+        const T r = residual<T, FORM>(p, H[c.h], Hold[c.h], div);
+        if (resh) resh[t] = r;
+        const double rd = (double)r;
+        s = rd * rd;
+        m = fabs(rd);
+    }
+    block_sum_max(s, m, threadIdx.x, kBlock, &partial[blockIdx.x]);
+}
+
+// compute_iter_params!: dtau_rho from the 3/5/7-point @maxloc(D).
+template <typename T>
+__global__ void __launch_bounds__(kBlock)
+k_iter_params(const KP<T> p, const T* __restrict__ D, T* __restrict__ dtau, int64_t count) {
+    const int64_t t = (int64_t)blockIdx.x * kBlock + threadIdx.x;
+    if (t >= count) return;
+    const CellIdx c = cell_index(p, t);
+    const int64_t h = c.h;
+    T m = fmax(fmax(D[h - 1], D[h + 1]), D[h]);
+    if (p.hy) m = fmax(m, fmax(D[h - p.nx], D[h + p.nx]));
+    if (p.hz) m = fmax(m, fmax(D[h - p.nx * p.ny], D[h + p.nx * p.ny]));
+    dtau[t] = step_dtau(p, m);
+}
+
+// Sum of the per-block partials in a fixed order (deterministic): out = (sum, max).
+__global__ void __launch_bounds__(kBlock)
+k_reduce_partials(const double2* __restrict__ partial, int64_t n, double* __restrict__ out) {
+    double s = 0.0, m = 0.0;
+    for (int64_t t = threadIdx.x; t < n; t += kBlock) {
+        const double2 v = partial[t];
+        s += v.x;
+        m = nanmax(v.y, m);
+    }
+    __shared__ double2 res;
+    block_sum_max(s, m, threadIdx.x, kBlock, &res);
+    __syncthreads();
+    if (threadIdx.x == 0) {
+        out[0] = res.x;
+        out[1] = res.y;
+    }
+}
+
+// ---------------------------------------------------------------------------------------------
+// fused 3D PT iteration
+// ---------------------------------------------------------------------------------------------
+//
+// Block = 64 lanes (x) x R rows (y); lane L of tile bx holds cell i = 62*bx + L and row r of tile by
+// holds j = (R-2)*by + r, so lanes 0/63 and rows 0/R-1 are one-cell overlaps with the neighbouring
+// tiles. Every active lane computes the flux on its own lower x face (i-1|i) and lower y face (j-1|j);
+// the upper faces of a cell are its neighbours' lower faces (shuffle down in x, LDS row+1 in y).
+// Updated cells: lanes 1..62, rows 1..R-2. z: the block marches k = k0..k1-1; the upper z face flux of
+// plane k is carried in a register as the lower z face of plane k+1. Loads for plane k+1 are issued
+// before plane k is computed. Each face and each cell is written by exactly one thread.
+// CHECK also evaluates the plain form's residual (residual fluxes from the pre-update H, transient
+// term from the updated H: scripts/diff_3D/diff_3D_nonlin_multixpu.jl:41-43,53) and reduces it.
+template <typename T, int DC, int FORM, int R, bool CHECK>
+__global__ void __launch_bounds__(64 * R)
+k_fused3d(const KP<T> p, const T* __restrict__ Hin, T* __restrict__ Hout, const T* __restrict__ Hold,
+          const T* __restrict__ D, const T* __restrict__ qxi, const T* __restrict__ qyi,
+          const T* __restrict__ qzi, T* __restrict__ qxo, T* __restrict__ qyo, T* __restrict__ qzo,
+          double2* __restrict__ partial, int tz, int ntx, int nty, int ntiles, int per_xcd) {
+    constexpr bool STEP = DC == PTD_STEP;
+    __shared__ T sH[R][64];
+    __shared__ T sQ[R][64];
+    __shared__ T sD[STEP ? R : 1][64];
+    __shared__ T sQ2[CHECK ? R : 1][64];
+
+    // XCD-aware tile order: consecutive blocks go round-robin to the 8 XCDs, so block b takes tile
+    // (b % 8) * per_xcd + b / 8 and each XCD walks one contiguous range of tiles (shared halos in L2).
+    const int b = blockIdx.x;
+    const int tile = (b & 7) * per_xcd + (b >> 3);
+    if (tile >= ntiles) return;
+    const int bx = tile % ntx;
+    const int rest = tile / ntx;
+    const int by = rest % nty;
+    const int bz = rest / nty;
+
+    const int lane = threadIdx.x, row = threadIdx.y;
+    const int64_t nx = p.nx, ny = p.ny, nz = p.nz;
+    const int64_t i = 62 * (int64_t)bx + lane;
+    const int64_t j = (int64_t)(R - 2) * by + row;
+    const int64_t k0 = 1 + (int64_t)tz * bz;
+    const int64_t k1 = (k0 + tz < nz - 1) ? k0 + tz : nz - 1;
+
+    const bool ld = (i <= nx - 1) && (j <= ny - 1);
+    const bool xin = (i >= 1) && (i <= nx - 2);
+    const bool yin = (j >= 1) && (j <= ny - 2);
+    const bool rmid = (row >= 1) && (row <= R - 2);
+    const bool lmid = (lane >= 1) && (lane <= 62);
+    const bool upd = lmid && rmid && xin && yin;
+    const bool fx = (lane >= 1) && (i <= nx - 1) && rmid && yin;
+    const bool fy = (row >= 1) && (j <= ny - 1) && lmid && xin;
+    const bool wx = fx && ((lane <= 62 && i <= nx - 2) || i == nx - 1);
+    const bool wy = fy && ((row <= R - 2 && j <= ny - 2) || j == ny - 1);
+
+    const int64_t psH = nx * ny, offH = i + nx * j;
+    const int64_t psQx = (nx - 1) * (ny - 2), offQx = (i - 1) + (nx - 1) * (j - 1);
+    const int64_t psQy = (nx - 2) * (ny - 1), offQy = (i - 1) + (nx - 2) * (j - 1);
+    const int64_t psQz = (nx - 2) * (ny - 2), offQz = (i - 1) + (nx - 2) * (j - 1);
+
+    // lower z face of the first plane
+    T Hc = load_or0(Hin, offH + psH * k0, ld);
+    T Dc = 0, Dkm = 0;
+    if constexpr (STEP) {
+        Dc = load_or0(D, offH + psH * k0, ld);
+        Dkm = load_or0(D, offH + psH * (k0 - 1), upd);
+    }
+    T qzL = 0, q2zL = 0;
+    if (upd) {
+        const T Hkm = Hin[offH + psH * (k0 - 1)];
+        const T coef = face_coef<T, DC, FORM>(p, Hkm, Hc, Dkm, Dc);
+        const T dH = Hc - Hkm;
+        const T th = face_theta<T, DC, FORM>(p, coef);
+        qzL = damped_flux<T, FORM>(qzi[offQz + psQz * (k0 - 1)], th, coef, dH, p.dz, p._dz);
+        if constexpr (CHECK) q2zL = res_flux<T, FORM>(coef, dH, p.dz, p._dz);
+        if (k0 == 1) qzo[offQz] = qzL;
+    }
+
+    // operands of plane k0
+    T nH = load_or0(Hin, offH + psH * (k0 + 1), ld);
+    T nD = 0;
+    if constexpr (STEP) nD = load_or0(D, offH + psH * (k0 + 1), ld);
+    T nHo = load_or0(Hold, offH + psH * k0, upd);
+    T nqx = load_or0(qxi, offQx + psQx * (k0 - 1), fx);
+    T nqy = load_or0(qyi, offQy + psQy * (k0 - 1), fy);
+    T nqz = load_or0(qzi, offQz + psQz * k0, upd);
+
+    double acc = 0.0, amax = 0.0;
+    for (int64_t k = k0; k < k1; ++k) {
+        const T Hn = nH, Dn = nD, Ho = nHo, qx_old = nqx, qy_old = nqy, qz_old = nqz;
+        if (k + 1 < k1) {  // prefetch plane k+1 (k+2 <= nz-1 here)
+            nH = load_or0(Hin, offH + psH * (k + 2), ld);
+            if constexpr (STEP) nD = load_or0(D, offH + psH * (k + 2), ld);
+            nHo = load_or0(Hold, offH + psH * (k + 1), upd);
+            nqx = load_or0(qxi, offQx + psQx * k, fx);
+            nqy = load_or0(qyi, offQy + psQy * k, fy);
+            nqz = load_or0(qzi, offQz + psQz * (k + 1), upd);
+        }
+        sH[row][lane] = Hc;
+        if constexpr (STEP) sD[row][lane] = Dc;
+        __syncthreads();
+
+        const T Hw = __shfl_up(Hc, 1);
+        const T Hs = row > 0 ? sH[row - 1][lane] : (T)0;
+        T Dw = 0, De = 0, Ds = 0, Dnn = 0;
+        if constexpr (STEP) {
+            Dw = __shfl_up(Dc, 1);
+            De = __shfl_down(Dc, 1);
+            Ds = row > 0 ? sD[row - 1][lane] : (T)0;
+            Dnn = row < R - 1 ? sD[row + 1][lane] : (T)0;
+        }
+        T qxn = 0, qyn = 0, qzn = 0, q2x = 0, q2y = 0, q2z = 0;
+        if (fx) {
+            const T coef = face_coef<T, DC, FORM>(p, Hw, Hc, Dw, Dc);
+            const T dH = Hc - Hw;
+            qxn = damped_flux<T, FORM>(qx_old, face_theta<T, DC, FORM>(p, coef), coef, dH, p.dx, p._dx);
+            if constexpr (CHECK) q2x = res_flux<T, FORM>(coef, dH, p.dx, p._dx);
+        }
+        if (fy) {
+            const T coef = face_coef<T, DC, FORM>(p, Hs, Hc, Ds, Dc);
+            const T dH = Hc - Hs;
+            qyn = damped_flux<T, FORM>(qy_old, face_theta<T, DC, FORM>(p, coef), coef, dH, p.dy, p._dy);
+            if constexpr (CHECK) q2y = res_flux<T, FORM>(coef, dH, p.dy, p._dy);
+        }
+        if (upd) {
+            const T coef = face_coef<T, DC, FORM>(p, Hc, Hn, Dc, Dn);
+            const T dH = Hn - Hc;
+            qzn = damped_flux<T, FORM>(qz_old, face_theta<T, DC, FORM>(p, coef), coef, dH, p.dz, p._dz);
+            if constexpr (CHECK) q2z = res_flux<T, FORM>(coef, dH, p.dz, p._dz);
+        }
+        sQ[row][lane] = qyn;
+        if constexpr (CHECK) sQ2[row][lane] = q2y;
+        __syncthreads();
+
+        const T qxE = __shfl_down(qxn, 1);
+        const T qyN = row < R - 1 ? sQ[row + 1][lane] : (T)0;
+        T q2xE = 0, q2yN = 0;
+        if constexpr (CHECK) {
+            q2xE = __shfl_down(q2x, 1);
+            q2yN = row < R - 1 ? sQ2[row + 1][lane] : (T)0;
+        }
+        if (upd) {
+            T dtr;
+            if constexpr (DC == PTD_LIN) {
+                dtr = p.dtau_rho;
+            } else if constexpr (STEP) {
+                T m = fmax(fmax(Dw, De), Dc);
+                m = fmax(m, fmax(Ds, Dnn));
+                m = fmax(m, fmax(Dkm, Dn));
+                dtr = step_dtau(p, m);
+            } else {
+                dtr = nonlin_dtau<T, FORM>(p, Hc);
+            }
+            const T div = divergence<T, FORM, 3>(p, qxE - qxn, qyN - qyn, qzn - qzL);
+            const T Hnew = h_update<T, FORM>(p, Hc, Ho, dtr, div);
+            Hout[offH + psH * k] = Hnew;
+            qzo[offQz + psQz * k] = qzn;
+            if constexpr (CHECK) {
+                const T div2 = divergence<T, FORM, 3>(p, q2xE - q2x, q2yN - q2y, q2z - q2zL);
+                const double r = (double)residual<T, FORM>(p, Hnew, Ho, div2);
+                acc += r * r;
+                amax = nanmax(fabs(r), amax);
+            }
+        }
+        if (wx) qxo[offQx + psQx * (k - 1)] = qxn;
+        if (wy) qyo[offQy + psQy * (k - 1)] = qyn;
+        qzL = qzn;
+        q2zL = q2z;
+        Hc = Hn;
+        if constexpr (STEP) {
+            Dkm = Dc;
+            Dc = Dn;
+        }
+    }
+    if constexpr (CHECK) block_sum_max(acc, amax, row * 64 + lane, 64 * R, &partial[tile]);
+}
+
+// ---------------------------------------------------------------------------------------------
+// launchers
+// ---------------------------------------------------------------------------------------------
+
+static inline unsigned grid_for(int64_t n) { return (unsigned)((n + kBlock - 1) / kBlock); }
+
+template <typename T, int DC, int FORM>
+static hipError_t flux_t(ptd_ctx* c, int q2mode) {
+    const KP<T> kp = make_kp<T>(c->p);
+    const T* H = (const T*)c->H[c->hcur];
+    const T* D = (const T*)c->D;
+    for (int a = 0; a < 3; ++a) {
+        const int64_t n = c->nQ[a];
+        if (n == 0) continue;
+        T* q = (T*)c->q[a][c->qcur];
+        T* q2 = (T*)c->q2[a];
+        const dim3 g(grid_for(n)), bl(kBlock);
+#define PTD_FLUX_AXIS(A)                                                                                      \
+    if (a == A) {                                                                                             \
+        if (q2mode == 0) hipLaunchKernelGGL((k_flux<T, DC, FORM, A, 0>), g, bl, 0, c->stream, kp, H, D, q, q2, n); \
+        else if (q2mode == 1) hipLaunchKernelGGL((k_flux<T, DC, FORM, A, 1>), g, bl, 0, c->stream, kp, H, D, q, q2, n); \
+        else hipLaunchKernelGGL((k_flux<T, DC, FORM, A, 2>), g, bl, 0, c->stream, kp, H, D, q, q2, n);         \
+    }
+        PTD_FLUX_AXIS(0)
+        PTD_FLUX_AXIS(1)
+        PTD_FLUX_AXIS(2)
+#undef PTD_FLUX_AXIS
+    }
+    return hipGetLastError();
+}
+
+template <typename T, int DC, int FORM>
+static hipError_t update_t(ptd_ctx* c) {
+    const KP<T> kp = make_kp<T>(c->p);
+    const dim3 g(grid_for(c->nR)), bl(kBlock);
+    T* H = (T*)c->H[c->hcur];
+    const T* Ho = (const T*)c->Hold;
+    const T* dt = (const T*)c->dtau;
+    const T* qx = (const T*)c->q[0][c->qcur];
+    const T* qy = (const T*)c->q[1][c->qcur];
+    const T* qz = (const T*)c->q[2][c->qcur];
+    switch (c->p.ndim) {
+        case 1: hipLaunchKernelGGL((k_update<T, DC, FORM, 1>), g, bl, 0, c->stream, kp, H, Ho, dt, qx, qy, qz, c->nR); break;
+        case 2: hipLaunchKernelGGL((k_update<T, DC, FORM, 2>), g, bl, 0, c->stream, kp, H, Ho, dt, qx, qy, qz, c->nR); break;
+        default: hipLaunchKernelGGL((k_update<T, DC, FORM, 3>), g, bl, 0, c->stream, kp, H, Ho, dt, qx, qy, qz, c->nR); break;
+    }
+    return hipGetLastError();
+}
+
+template <typename T, int FORM>
+static hipError_t residual_t(ptd_ctx* c, bool write_resh) {
+    const KP<T> kp = make_kp<T>(c->p);
+    const unsigned nb = grid_for(c->nR);
+    const dim3 g(nb), bl(kBlock);
+    const T* H = (const T*)c->H[c->hcur];
+    const T* Ho = (const T*)c->Hold;
+    const T* q0 = (const T*)c->q2[0];
+    const T* q1 = (const T*)c->q2[1];
+    const T* q2 = (const T*)c->q2[2];
+    T* r = write_resh ? (T*)c->resh : nullptr;
+    switch (c->p.ndim) {
+        case 1: hipLaunchKernelGGL((k_residual<T, FORM, 1>), g, bl, 0, c->stream, kp, H, Ho, q0, q1, q2, r, (double2*)c->partial, c->nR); break;
+        case 2: hipLaunchKernelGGL((k_residual<T, FORM, 2>), g, bl, 0, c->stream, kp, H, Ho, q0, q1, q2, r, (double2*)c->partial, c->nR); break;
+        default: hipLaunchKernelGGL((k_residual<T, FORM, 3>), g, bl, 0, c->stream, kp, H, Ho, q0, q1, q2, r, (double2*)c->partial, c->nR); break;
+    }
+    hipLaunchKernelGGL(k_reduce_partials, dim3(1), dim3(kBlock), 0, c->stream, (const double2*)c->partial,
+                       (int64_t)nb, c->dred);
+    return hipGetLastError();
+}
+
+static constexpr int kRows = 16;  // rows (waves) per fused 3D block
+
+static int fused_tz(const ptd_ctx* c) {
+    const char* env = getenv("PTD_TZ");
+    if (env && atoi(env) > 0) return atoi(env);
+    const int64_t ntx = (c->p.nx - 2 + 61) / 62, nty = (c->p.ny - 2 + kRows - 3) / (kRows - 2);
+    const int64_t mz = c->p.nz - 2;
+    // enough z chunks for ~4 blocks per CU on 256 CUs, chunks of at least 16 planes
+    int64_t want = std::max<int64_t>(1, (1024 + ntx * nty - 1) / (ntx * nty));
+    int64_t tz = std::max<int64_t>(16, (mz + want - 1) / want);
+    return (int)std::min<int64_t>(tz, mz);
+}
+
+template <typename T, int DC, int FORM>
+static hipError_t fused_t(ptd_ctx* c, bool check) {
+    const KP<T> kp = make_kp<T>(c->p);
+    const int tz = fused_tz(c);
+    const int ntx = (int)((c->p.nx - 2 + 61) / 62);
+    const int nty = (int)((c->p.ny - 2 + kRows - 3) / (kRows - 2));
+    const int nbz = (int)((c->p.nz - 2 + tz - 1) / tz);
+    const int ntiles = ntx * nty * nbz;
+    const int per = (ntiles + 7) / 8;
+    const dim3 g(8 * per), bl(64, kRows);
+    const int h = c->hcur, q = c->qcur;
+    const T* Hin = (const T*)c->H[h];
+    T* Hout = (T*)c->H[1 - h];
+    const T* Ho = (const T*)c->Hold;
+    const T* D = (const T*)c->D;
+    if (check)
+        hipLaunchKernelGGL((k_fused3d<T, DC, FORM, kRows, true>), g, bl, 0, c->stream, kp, Hin, Hout, Ho, D,
+                           (const T*)c->q[0][q], (const T*)c->q[1][q], (const T*)c->q[2][q], (T*)c->q[0][1 - q],
+                           (T*)c->q[1][1 - q], (T*)c->q[2][1 - q], (double2*)c->partial, tz, ntx, nty, ntiles, per);
+    else
+        hipLaunchKernelGGL((k_fused3d<T, DC, FORM, kRows, false>), g, bl, 0, c->stream, kp, Hin, Hout, Ho, D,
+                           (const T*)c->q[0][q], (const T*)c->q[1][q], (const T*)c->q[2][q], (T*)c->q[0][1 - q],
+                           (T*)c->q[1][1 - q], (T*)c->q[2][1 - q], (double2*)c->partial, tz, ntx, nty, ntiles, per);
+    hipError_t e = hipGetLastError();
+    if (e != hipSuccess) return e;
+    c->hcur = 1 - h;
+    c->qcur = 1 - q;
+    if (check)
+        hipLaunchKernelGGL(k_reduce_partials, dim3(1), dim3(kBlock), 0, c->stream, (const double2*)c->partial,
+                           (int64_t)ntiles, c->dred);
+    return hipGetLastError();
+}
+
+// dispatch on (precision, dcase, form)
+#define PTD_DISPATCH(FN, c, ...)                                                                        \
+    do {                                                                                                \
+        const bool f32 = (c)->esz == 4;                                                                 \
+        const int dc = (c)->p.dcase, fm = (c)->p.form;                                                  \
+        if (f32) {                                                                                      \
+            if (dc == PTD_LIN) return fm == PTD_PERF ? FN<float, PTD_LIN, PTD_PERF>(c, ##__VA_ARGS__)    \
+                                                     : FN<float, PTD_LIN, PTD_PLAIN>(c, ##__VA_ARGS__);  \
+            if (dc == PTD_STEP) return fm == PTD_PERF ? FN<float, PTD_STEP, PTD_PERF>(c, ##__VA_ARGS__)  \
+                                                      : FN<float, PTD_STEP, PTD_PLAIN>(c, ##__VA_ARGS__); \
+            return fm == PTD_PERF ? FN<float, PTD_NONLIN, PTD_PERF>(c, ##__VA_ARGS__)                     \
+                                  : FN<float, PTD_NONLIN, PTD_PLAIN>(c, ##__VA_ARGS__);                   \
+        }                                                                                               \
+        if (dc == PTD_LIN) return fm == PTD_PERF ? FN<double, PTD_LIN, PTD_PERF>(c, ##__VA_ARGS__)       \
+                                                 : FN<double, PTD_LIN, PTD_PLAIN>(c, ##__VA_ARGS__);     \
+        if (dc == PTD_STEP) return fm == PTD_PERF ? FN<double, PTD_STEP, PTD_PERF>(c, ##__VA_ARGS__)     \
+                                                  : FN<double, PTD_STEP, PTD_PLAIN>(c, ##__VA_ARGS__);   \
+        return fm == PTD_PERF ? FN<double, PTD_NONLIN, PTD_PERF>(c, ##__VA_ARGS__)                        \
+                              : FN<double, PTD_NONLIN, PTD_PLAIN>(c, ##__VA_ARGS__);                      \
+    } while (0)
+
+hipError_t launch_flux(ptd_ctx* c, int q2mode) { PTD_DISPATCH(flux_t, c, q2mode); }
+hipError_t launch_update(ptd_ctx* c) { PTD_DISPATCH(update_t, c); }
+hipError_t launch_fused(ptd_ctx* c, bool check) { PTD_DISPATCH(fused_t, c, check); }
+
+hipError_t launch_residual(ptd_ctx* c, bool write_resh) {
+    const bool f32 = c->esz == 4;
+    if (c->p.form == PTD_PERF) return f32 ? residual_t<float, PTD_PERF>(c, write_resh) : residual_t<double, PTD_PERF>(c, write_resh);
+    return f32 ? residual_t<float, PTD_PLAIN>(c, write_resh) : residual_t<double, PTD_PLAIN>(c, write_resh);
+}
+
+hipError_t launch_iter_params(ptd_ctx* c) {
+    const dim3 g(grid_for(c->nR)), bl(kBlock);
+    if (c->esz == 4) {
+        const KP<float> kp = make_kp<float>(c->p);
+        hipLaunchKernelGGL(k_iter_params<float>, g, bl, 0, c->stream, kp, (const float*)c->D, (float*)c->dtau, c->nR);
+    } else {
+        const KP<double> kp = make_kp<double>(c->p);
+        hipLaunchKernelGGL(k_iter_params<double>, g, bl, 0, c->stream, kp, (const double*)c->D, (double*)c->dtau, c->nR);
+    }
+    return hipGetLastError();
+}
+
+bool fused_available(const ptd_ctx* c) {
+    return c->p.ndim == 3 && c->p.nx >= 3 && c->p.ny >= 3 && c->p.nz >= 3;
+}
+
+// Number of (sum,max) partials the reductions can produce for this context.
+int64_t partial_capacity(const ptd_params& p, int64_t nR) {
+    int64_t nb = (nR + kBlock - 1) / kBlock;
+    if (p.ndim == 3) {
+        const int64_t ntx = (p.nx - 2 + 61) / 62, nty = (p.ny - 2 + kRows - 3) / (kRows - 2);
+        // smallest chunk is one plane
+        nb = std::max<int64_t>(nb, ntx * nty * (p.nz - 2));
+    }
+    return std::max<int64_t>(nb, 1);
+}
+
+}  // namespace ptd

@@ -1,0 +1,56 @@
+"""The C-ABI library loads, exports every function include/ptd.h declares, and validates arguments
+before touching a device (no GPU needed)."""
+import ctypes
+import os
+import re
+
+import pytest
+
+from conftest import ROOT
+
+
+def _declared_functions():
+    text = open(os.path.join(ROOT, "include", "ptd.h")).read()
+    text = re.sub(r"/\*.*?\*/", "", text, flags=re.S)
+    return sorted(set(re.findall(r"\b(ptd_[a-z0-9_]+)\s*\(", text)))
+
+
+def test_library_exports_every_declared_function(pkg):
+    lib = pkg.load()
+    declared = _declared_functions()
+    assert len(declared) >= 20
+    from importlib import import_module
+    abi = import_module("ptd_amd._abi")
+    assert sorted(abi.EXPORTS) == declared
+    for name in declared:
+        assert hasattr(lib, name), f"{name} not exported by libptd.so"
+
+
+def test_abi_version(pkg):
+    assert pkg.load().ptd_abi_version() == 1
+
+
+def test_struct_layout_matches_header(pkg):
+    # ptd_params: 4 int32, 4 int64, 2 int32, 14 doubles
+    assert ctypes.sizeof(pkg.Params) == 4 * 4 + 4 * 8 + 2 * 4 + 14 * 8
+
+
+@pytest.mark.parametrize("field,value", [("ndim", 4), ("dcase", 7), ("form", 3), ("precision", 9),
+                                         ("nx", 2), ("nout", 0), ("dx", 0.0), ("itmax", -1)])
+def test_invalid_params_rejected_before_device(field, value, pkg):
+    from importlib import import_module
+    abi = import_module("ptd_amd._abi")
+    p = pkg.make_params(pkg.SCRIPTS["diff_3D_nonlin_perf"], 16, 16, 16)
+    setattr(p, field, value)
+    with pytest.raises(pkg.PTDError) as e:
+        pkg.Context(p)
+    assert e.value.status == abi.ERR_ARG
+
+
+def test_null_context_is_an_argument_error(pkg):
+    lib = pkg.load()
+    from importlib import import_module
+    abi = import_module("ptd_amd._abi")
+    assert lib.ptd_iterate(None, 1) == abi.ERR_ARG
+    assert lib.ptd_check_res(None, None, None) == abi.ERR_ARG
+    assert b"NULL" in lib.ptd_last_error()

@@ -1,0 +1,256 @@
+"""GPU parity: libptd (HIP, gfx950) through the C ABI against the CPU oracle and the reference goldens.
+
+Tolerances (float64): the device kernels evaluate every expression in the reference's operation order
+with IEEE division/sqrt and no contraction, so results are expected to be bit-identical to the oracle;
+the tests assert a relative max-norm difference <= 1e-12 (tighter than the 1e-10 target) and report
+the number of bit-identical entries. Iteration counts must be equal. Norms are reduced in a different
+order than the oracle: errors agree to 1e-12 relative.
+float32: relative max-norm difference to the float64 oracle <= 1e-4 after short runs.
+"""
+import numpy as np
+import pytest
+
+from conftest import FIXTURE_SCRIPTS, isapprox_norm, sample
+
+pytestmark = pytest.mark.gpu
+
+RTOL64 = 1e-12
+
+
+def _relmax(a, b):
+    return float(np.max(np.abs(a - b)) / max(np.max(np.abs(b)), 1e-300))
+
+
+@pytest.fixture(scope="module", autouse=True)
+def _device(pkg):
+    lib = pkg.load()
+    st = lib.ptd_device_check(0, None, 0)
+    if st != 0:
+        pytest.fail(f"no gfx950 device: {lib.ptd_last_error()}")
+
+
+# --------------------------------------------------------------------------------------------------
+# the reference's reftests, run through the GPU path
+# --------------------------------------------------------------------------------------------------
+
+@pytest.mark.parametrize("fixture", sorted(FIXTURE_SCRIPTS))
+def test_gpu_reproduces_reftest_and_oracle(fixture, pkg, orc, reftests):
+    script, n = FIXTURE_SCRIPTS[fixture]
+    print(f"[gpu] reftest {fixture} n={n}", flush=True)
+    res = pkg.run_script(script, n, n, n)
+    ref = orc.run_script(script, n, n, n)
+    nd = ref["case"].ndim
+    Hs, inds = sample(res.H, n, nd)
+    assert np.array_equal(res.coords[0][inds], reftests[fixture]["X"])
+    assert isapprox_norm(Hs, reftests[fixture]["H"])
+    assert np.linalg.norm(Hs - reftests[fixture]["H"]) / np.linalg.norm(reftests[fixture]["H"]) <= 1e-13
+    assert list(res.iters) == list(ref["iters"])
+    assert _relmax(res.H, ref["H"]) <= RTOL64
+    print(f"[gpu]   identical entries {np.sum(res.H == ref['H'])}/{res.H.size}", flush=True)
+
+
+# --------------------------------------------------------------------------------------------------
+# kernel-level entry points (one reference kernel each) on random states
+# --------------------------------------------------------------------------------------------------
+
+def _random_state(pkg, orc, script, shape, seed):
+    rng = np.random.default_rng(seed)
+    spec = pkg.SCRIPTS[script]
+    nx, ny, nz = (list(shape) + [1, 1])[:3]
+    params = pkg.make_params(spec, nx, ny, nz)
+    case = orc.make_case(orc.SCRIPTS[script], nx, ny, nz)
+    H = np.asfortranarray(rng.uniform(0.5, 2.0, size=shape))
+    Hold = np.asfortranarray(rng.uniform(0.5, 2.0, size=shape))
+    D = np.asfortranarray(np.where(rng.uniform(size=shape) < 0.5, 1e-4, 1.0)) if spec["dcase"] == pkg.STEP else None
+    q = {}
+    for name in ("qHx", "qHy", "qHz")[:len(shape)]:
+        q[name] = np.asfortranarray(rng.normal(size=pkg.field_shape(params, {"qHx": 3, "qHy": 4, "qHz": 5}[name])))
+    return params, case, H, Hold, D, q
+
+
+KERNEL_CASES = [
+    ("diff_1D_lin", (23,)), ("diff_1D_linstep", (23,)), ("diff_1D_nonlin", (23,)),
+    ("diff_2D_lin", (19, 13)), ("diff_2D_linstep", (19, 13)), ("diff_2D_nonlin", (19, 13)),
+    ("diff_2D_nonlin_perf", (19, 13)),
+    ("diff_3D_lin_multixpu", (11, 9, 7)), ("diff_3D_linstep_multixpu", (11, 9, 7)),
+    ("diff_3D_nonlin_multixpu", (11, 9, 7)), ("diff_3D_nonlin_perf", (11, 9, 7)),
+]
+
+
+@pytest.mark.parametrize("script,shape", KERNEL_CASES)
+def test_kernel_level_entry_points(script, shape, pkg, orc):
+    print(f"[gpu] kernels {script} {shape}", flush=True)
+    params, case, H, Hold, D, q = _random_state(pkg, orc, script, shape, seed=len(script))
+    nd = len(shape)
+    qn = ("qHx", "qHy", "qHz")[:nd]
+    with pkg.Context(params) as ctx:
+        ctx.set("H", H)
+        ctx.set("Hold", Hold)
+        for k, v in q.items():
+            ctx.set(k, v)
+        dtau = None
+        if D is not None:
+            ctx.set("D", D)
+            ctx.compute_iter_params()
+            dtau = np.zeros(pkg.field_shape(params, 10), order="F")
+            orc.lib().ptd_iter_params(orc.ctypes.byref(case), orc._p(D), orc._p(dtau))
+            assert np.array_equal(ctx.get("dtau_rho"), dtau)
+        # compute_flux!
+        ctx.compute_flux()
+        qo = {k: v.copy(order="F") for k, v in q.items()}
+        q2o = {k: np.zeros_like(v) for k, v in q.items()}
+        args = [qo.get(k) for k in ("qHx", "qHy", "qHz")] + [q2o.get(k) for k in ("qHx", "qHy", "qHz")]
+        orc.lib().ptd_flux(orc.ctypes.byref(case), orc._p(H), orc._p(D), *[orc._p(a) for a in args])
+        for k in qn:
+            assert np.array_equal(ctx.get(k), qo[k]), k
+            if params.form == pkg.PLAIN:
+                assert np.array_equal(ctx.get(k + "2"), q2o[k]), k + "2"
+        # compute_update!
+        ctx.compute_update()
+        Hr = H.copy(order="F")
+        orc.lib().ptd_update(orc.ctypes.byref(case), orc._p(Hr), orc._p(Hold), orc._p(dtau),
+                             *[orc._p(qo.get(k)) for k in ("qHx", "qHy", "qHz")])
+        assert np.array_equal(ctx.get("H"), Hr)
+        # compute_flux_res! + check_res!
+        ctx.compute_flux_res()
+        orc.lib().ptd_flux_res(orc.ctypes.byref(case), orc._p(Hr), orc._p(D),
+                               *[orc._p(q2o.get(k)) for k in ("qHx", "qHy", "qHz")])
+        for k in qn:
+            assert np.array_equal(ctx.get(k + "2"), q2o[k])
+        rms, mx = ctx.check_res()
+        resh = np.zeros(pkg.field_shape(params, 9), order="F")
+        emax = orc.ctypes.c_double()
+        ref_rms = orc.lib().ptd_check_res(orc.ctypes.byref(case), orc._p(Hr), orc._p(Hold),
+                                          *[orc._p(q2o.get(k)) for k in ("qHx", "qHy", "qHz")],
+                                          orc._p(resh), orc.ctypes.byref(emax))
+        assert np.array_equal(ctx.get("ResH"), resh)
+        assert rms == pytest.approx(ref_rms, rel=1e-12) and mx == emax.value
+        # assign!
+        ctx.assign()
+        assert np.array_equal(ctx.get("Hold"), Hr)
+
+
+# --------------------------------------------------------------------------------------------------
+# fused 3D sweep on ragged shapes: several x / y tiles, partial tiles, several z chunks
+# --------------------------------------------------------------------------------------------------
+
+FUSED_SHAPES = [(37, 29, 21), (70, 33, 18), (131, 19, 47), (3, 3, 3), (64, 16, 5)]
+FUSED_SCRIPTS = ["diff_3D_lin_multixpu", "diff_3D_linstep_multixpu", "diff_3D_nonlin_multixpu",
+                 "diff_3D_nonlin_perf"]
+
+
+@pytest.mark.parametrize("shape", FUSED_SHAPES)
+@pytest.mark.parametrize("script", FUSED_SCRIPTS)
+def test_fused_sweeps_match_oracle(script, shape, pkg, orc, monkeypatch):
+    print(f"[gpu] fused {script} {shape}", flush=True)
+    monkeypatch.setenv("PTD_TZ", "4")  # several z chunks per column
+    params, case, H, Hold, D, q = _random_state(pkg, orc, script, shape, seed=sum(shape))
+    niter = 5
+    with pkg.Context(params) as ctx:
+        ctx.set("H", H)
+        ctx.set("Hold", Hold)
+        for k, v in q.items():
+            ctx.set(k, v)
+        dtau = None
+        if D is not None:
+            ctx.set("D", D)
+            dtau = np.zeros(pkg.field_shape(params, 10), order="F")
+            orc.lib().ptd_iter_params(orc.ctypes.byref(case), orc._p(D), orc._p(dtau))
+        ctx.iterate(niter)
+        Hg = ctx.get("H")
+        qg = {k: ctx.get(k) for k in q}
+    Hr = H.copy(order="F")
+    qr = {k: v.copy(order="F") for k, v in q.items()}
+    orc.lib().ptd_iterate(orc.ctypes.byref(case), orc._p(Hr), orc._p(Hold), orc._p(D), orc._p(dtau),
+                          orc._p(qr["qHx"]), orc._p(qr["qHy"]), orc._p(qr["qHz"]), niter)
+    assert np.array_equal(Hg, Hr), f"H: rel {_relmax(Hg, Hr):.2e}"
+    for k in q:
+        assert np.array_equal(qg[k], qr[k]), f"{k}: rel {_relmax(qg[k], qr[k]):.2e}"
+
+
+@pytest.mark.parametrize("script", ["diff_3D_nonlin_multixpu", "diff_3D_linstep_multixpu", "diff_3D_lin_multixpu"])
+def test_fused_solve_with_checks_matches_oracle(script, pkg, orc):
+    """Converging solve at 45x38x29: residual checks fused into the sweep (plain form)."""
+    shape = (45, 38, 29)
+    print(f"[gpu] solve {script} {shape}", flush=True)
+    res = pkg.run_script(script, *shape)
+    ref = orc.run_script(script, *shape)
+    assert list(res.iters) == list(ref["iters"])
+    assert _relmax(res.H, ref["H"]) <= RTOL64
+    np.testing.assert_allclose(res.errs, ref["errs"], rtol=1e-12)
+
+
+def test_perf_form_solve_with_checks(pkg, orc):
+    """Perf-form expressions with a converging loop (nout=10): residual from the post-update H."""
+    over = dict(nout=10, itMax=1e5)
+    res = pkg.run_script("diff_3D_nonlin_perf", 33, 30, 27, **over)
+    ref = orc.run_script("diff_3D_nonlin_perf", 33, 30, 27, nout=10, itmax=100000)
+    assert list(res.iters) == list(ref["iters"])
+    assert _relmax(res.H, ref["H"]) <= RTOL64
+
+
+def test_max_norm_stopping(pkg, orc):
+    res = pkg.run_script("diff_2D_nonlin", 48, 48, stop_norm=1)
+    assert all(e <= 1e-8 for e in res.errs)
+    base = pkg.run_script("diff_2D_nonlin", 48, 48)
+    assert res.ittot >= base.ittot  # max|r| >= rms(r)
+
+
+# --------------------------------------------------------------------------------------------------
+# float32 sweeps
+# --------------------------------------------------------------------------------------------------
+
+@pytest.mark.parametrize("script", ["diff_3D_nonlin_perf", "diff_2D_nonlin_perf", "diff_3D_linstep_multixpu"])
+def test_fp32_sweeps_close_to_fp64_oracle(script, pkg, orc):
+    n = 32
+    over = {"itMax": 100, "nout": 2000} if "linstep" in script else {"itMax": 100}
+    res = pkg.run_script(script, n, n, n, precision=pkg.FP32, ic="config" if "nonlin" in script else None, **over)
+    spec = dict(orc.SCRIPTS[script])
+    spec.update(itmax=100, nout=2000)
+    case = orc.make_case(spec, n, n, n)
+    H0 = pkg.initial_condition(pkg.SCRIPTS[script], n, n, n, kind="config" if "nonlin" in script else None)
+    H, Hold = H0.copy(order="F"), H0.copy(order="F")
+    D = pkg.step_field(pkg.SCRIPTS[script], n, n, n) if "linstep" in script else None
+    orc.solve(case, H, Hold, D)
+    assert _relmax(res.H, H) <= 1e-4
+
+
+# --------------------------------------------------------------------------------------------------
+# full-size properties (511^3, 1 x fused sweeps)
+# --------------------------------------------------------------------------------------------------
+
+def test_full_size_sweeps_properties(pkg, orc):
+    """511^3 nonlinear, config IC: boundaries unchanged, bitwise x<->y symmetry, finite; and a 2-plane
+    z-slab comparison against the oracle on a 255^3 run."""
+    n = 511
+    print(f"[gpu] full size {n}^3", flush=True)
+    spec = pkg.SCRIPTS["diff_3D_nonlin_perf"]
+    params = pkg.make_params(spec, n, n, n)
+    H0 = pkg.initial_condition(spec, n, n, n, kind="config")
+    with pkg.Context(params) as ctx:
+        ctx.set("H", H0)
+        ctx.set("Hold", H0)
+        ctx.iterate(20)
+        H = ctx.get("H")
+    assert np.all(np.isfinite(H))
+    for d in range(3):
+        for edge in (0, n - 1):
+            sl = [slice(None)] * 3
+            sl[d] = edge
+            assert np.array_equal(H[tuple(sl)], H0[tuple(sl)])
+    assert np.array_equal(H, np.swapaxes(H, 0, 1))  # (dx-term + dy-term) + dz-term: x<->y is exact
+    del H, H0
+    n = 255
+    params = pkg.make_params(spec, n, n, n)
+    H0 = pkg.initial_condition(spec, n, n, n, kind="config")
+    with pkg.Context(params) as ctx:
+        ctx.set("H", H0)
+        ctx.set("Hold", H0)
+        ctx.iterate(3)
+        Hg = ctx.get("H")
+    case = orc.make_case(orc.SCRIPTS["diff_3D_nonlin_perf"], n, n, n)
+    Hr, Ho = H0.copy(order="F"), H0.copy(order="F")
+    z = [np.zeros(s, order="F") for s in ((n - 1, n - 2, n - 2), (n - 2, n - 1, n - 2), (n - 2, n - 2, n - 1))]
+    orc.lib().ptd_iterate(orc.ctypes.byref(case), orc._p(Hr), orc._p(Ho), None, None,
+                          orc._p(z[0]), orc._p(z[1]), orc._p(z[2]), 3)
+    assert np.array_equal(Hg, Hr)

@@ -1,0 +1,94 @@
+"""The CPU oracle against the reference's own goldens: the 14 reftest BSON fixtures and the published
+iteration-count tables (output/out_diff_*.txt)."""
+import math
+
+import numpy as np
+import pytest
+
+from conftest import FIXTURE_SCRIPTS, isapprox_norm, sample
+
+# Oracle vs fixture: the fixtures are full-precision Float64; the oracle reproduces them to round-off.
+FIXTURE_RTOL = 1e-14
+
+
+@pytest.mark.parametrize("fixture", sorted(FIXTURE_SCRIPTS))
+def test_oracle_reproduces_reftest(fixture, orc, reftests):
+    script, n = FIXTURE_SCRIPTS[fixture]
+    r = orc.run_script(script, n, n, n)
+    nd = r["case"].ndim
+    Hs, inds = sample(r["H"], n, nd)
+    ref = reftests[fixture]
+    assert np.array_equal(r["xc"][inds], ref["X"]), "X (cell centres via LinRange) must match bit for bit"
+    assert Hs.shape == ref["H"].shape
+    assert isapprox_norm(Hs, ref["H"])                           # the reference's own comparison
+    rel = np.linalg.norm(Hs - ref["H"]) / np.linalg.norm(ref["H"])
+    assert rel <= FIXTURE_RTOL, f"{fixture}: relative difference {rel:.3e}"
+    assert r["status"] == 0
+
+
+def _published_cases():
+    out = []
+    for key, lim in (("1D_lin", 1024), ("1D_linstep", 1024), ("1D_nonlin", 1024),
+                     ("2D_lin", 128), ("2D_linstep", 128), ("2D_nonlin", 128),
+                     ("3D_lin", 62), ("3D_linstep", 62), ("3D_nonlin", 62)):
+        out.append((key, lim))
+    return out
+
+
+@pytest.mark.parametrize("key,limit", _published_cases())
+def test_oracle_published_iteration_counts(key, limit, orc, iteration_tables):
+    nd = int(key[0])
+    case = key.split("_", 1)[1]
+    script = f"diff_{nd}D_{case}" + ("_multixpu" if nd == 3 else "")
+    rows = [r for r in iteration_tables[key]["rows"] if r["n"][0] <= limit]
+    assert rows
+    for row in rows:
+        n = row["n"][0]
+        over = {}
+        if nd == 3:
+            # published 3D runs used 8 ranks (dims 2x2x2, scripts/diff_3D/runme.sh:41): local n = (n-2)/2+2
+            # and err normalised by len_ResH_g = ((nloc-4)*2+2)^3 (diff_3D_nonlin_multixpu.jl:89)
+            nloc = (n - 2) // 2 + 2
+            over["norm_len"] = float(((nloc - 4) * 2 + 2) ** 3)
+        r = orc.run_script(script, n, n, n, **over)
+        assert int(r["iters"].sum()) == row["ittot"], f"{key} n={n}"
+        assert len(r["iters"]) == row["nt"]
+        assert all(int(v) % 10 == 0 for v in r["iters"])  # checks every nout = 10
+
+
+def test_goldens_detect_planted_mistakes(orc, reftests, iteration_tables):
+    """Plausible mistakes planted in the oracle's parameters fail a golden. The fixtures hold converged
+    backward-Euler solutions, so they pin the discretisation (dt, D, IC); the published iteration
+    counts pin the PT dynamics (Resc, CFL, Re, nout)."""
+    fixture = "test_diff_3D_nonlin_perf"
+    script, n = FIXTURE_SCRIPTS[fixture]
+    ref = reftests[fixture]["H"]
+    for over in (dict(dt=0.25), dict(lx=10.5, ly=10.5, lz=10.5)):
+        r = orc.run_script(script, n, n, n, **over)
+        Hs, _ = sample(r["H"], n, 3)
+        rel = np.linalg.norm(Hs - ref) / np.linalg.norm(ref)
+        assert rel > 1e-6, f"planted {over} not detected by {fixture} (rel {rel:.2e})"
+    r = orc.run_script("diff_1D_lin", 64, d0=1.05)
+    Hs, _ = sample(r["H"], 64, 1)
+    assert np.linalg.norm(Hs - reftests["test_diff_1D_lin"]["H"]) > 1e-6
+    published = {row["n"][0]: row["ittot"] for row in iteration_tables["2D_nonlin"]["rows"]}
+    for over in (dict(resc=1.0), dict(cfl=1 / math.sqrt(3)), dict(nout=20)):
+        r = orc.run_script("diff_2D_nonlin", 128, 128, **over)
+        assert int(r["iters"].sum()) != published[128], f"planted {over} not detected by the iteration count"
+
+
+def test_fixture_comparison_detects_wrong_maxloc(orc, reftests):
+    """@maxloc is the 7-point neighbourhood: a centre-only dtau_rho (planted through a D field whose
+    step is moved by one cell) changes the linstep fixture beyond round-off."""
+    fixture = "test_diff_3D_linstep_multixpu"
+    n = 32
+    spec = dict(orc.SCRIPTS["diff_3D_linstep_multixpu"])
+    case = orc.make_case(spec, n, n, n)
+    H0 = orc.initial_condition(spec["ic"], case)
+    D = orc.step_field(1, case)
+    D = np.asfortranarray(np.roll(D, 1, axis=0))
+    H, Hold = H0.copy(order="F"), H0.copy(order="F")
+    orc.solve(case, H, Hold, D)
+    Hs, _ = sample(H, n, 3)
+    rel = np.linalg.norm(Hs - reftests[fixture]["H"]) / np.linalg.norm(reftests[fixture]["H"])
+    assert rel > 1e-8
