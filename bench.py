@@ -67,6 +67,9 @@ def main():
     ap.add_argument("--warmup", type=int, default=20)
     ap.add_argument("--n", type=int, default=511)
     ap.add_argument("--precision", choices=["fp64", "fp32"], default="fp64")
+    ap.add_argument("--arith", choices=["fast", "exact"], default="fast",
+                    help="fast: rearranged update rules (< 1e-10 from the reference on converged solutions); "
+                         "exact: reference operation order, bit-identical")
     ap.add_argument("--no-cpu", action="store_true", help="skip the CPU baseline leg")
     ap.add_argument("--no-tts", action="store_true", help="skip the time-to-solution leg")
     ap.add_argument("--cpu-seconds", type=float, default=15.0)
@@ -91,7 +94,8 @@ def main():
     prec = pkg.FP64 if args.precision == "fp64" else pkg.FP32
     esz = 8 if prec == pkg.FP64 else 4
     spec = pkg.SCRIPTS["diff_3D_nonlin_perf"]
-    params = pkg.make_params(spec, n, n, n, precision=prec)
+    arith = 1 if args.arith == "fast" else 0
+    params = pkg.make_params(spec, n, n, n, precision=prec, arith=arith)
     log(f"building config IC at {n}^3")
     H0 = pkg.initial_condition(spec, n, n, n, kind="config", amp=2.0, bval=1.0)
     a_eff = pkg.a_eff_gb(3, (n, n, n), esz)
@@ -107,6 +111,14 @@ def main():
         ms = ctx.iterate_timed(args.steps)
         sync()
         wall = time.perf_counter() - t0
+        if args.arith == "fast":
+            # the same sweep in the reference's exact operation order, for comparison
+            p_exact = pkg.make_params(spec, n, n, n, precision=prec, arith=0)
+            ctx.set_params(p_exact)
+            ctx.iterate(2)
+            ms_exact = ctx.iterate_timed(max(10, args.steps // 4)) / max(10, args.steps // 4)
+        else:
+            ms_exact = None
     t_it = ms / 1e3 / args.steps
     t_eff = a_eff / t_it
     log(f"t_it = {t_it * 1e3:.4f} ms (events), wall {wall / args.steps * 1e3:.4f} ms, T_eff = {t_eff:.1f} GB/s")
@@ -126,15 +138,18 @@ def main():
         "data": "synthetic",
         "config": {"workload": f"3D nonlinear D=H^3 PT iteration (fused flux+update sweep), {n}^3, "
                                f"perf-script expressions, H0=1+2exp(-r^2), H=1 boundaries",
-                   "nx": n, "ny": n, "nz": n, "A_eff_GB_per_iter": round(a_eff, 6)},
+                   "nx": n, "ny": n, "nz": n, "A_eff_GB_per_iter": round(a_eff, 6), "arith": args.arith},
         "roofline": {"bound": "hbm", "achieved": round(t_eff, 2), "peak": PEAK_HBM_GBS, "unit": "GB/s",
                      "frac": round(t_eff / PEAK_HBM_GBS, 4), "traffic": None},
     }
 
+    if ms_exact is not None:
+        out["exact_arith"] = {"ms_per_step": round(ms_exact, 5), "T_eff": round(a_eff / (ms_exact / 1e3), 2)}
+
     if not args.no_tts:
         # time-to-solution: converging solve, 5 physical steps, tol 1e-8 on the RMS residual, nout 10
         sp = dict(pkg.SCRIPTS["diff_3D_nonlin_multixpu"])
-        p2 = pkg.make_params(sp, n, n, n, precision=pkg.FP64)
+        p2 = pkg.make_params(sp, n, n, n, precision=pkg.FP64, arith=arith)
         nt = int(math.ceil(sp["ttot"] / sp["dt"]))
         with pkg.Context(p2) as ctx:
             ctx.set("H", H0)

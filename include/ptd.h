@@ -58,6 +58,15 @@ typedef enum ptd_form { PTD_PLAIN = 0, PTD_PERF = 1 } ptd_form;
  *             backward-Euler system and the accumulated solution stay float64 on the device. */
 typedef enum ptd_precision { PTD_FP64 = 0, PTD_FP32 = 1, PTD_MIXED = 2 } ptd_precision;
 
+/* Arithmetic of the sweeps.
+ *  PTD_ARITH_EXACT: every expression in the reference's operation order with IEEE division and
+ *                   square root (bit-identical to the reference kernels' Float64 results).
+ *  PTD_ARITH_FAST : the same update rules rearranged algebraically (one reciprocal per face and per
+ *                   cell instead of up to four divisions, Newton-refined hardware reciprocal and
+ *                   reciprocal square root); agrees with EXACT to a few ulp per iteration and to
+ *                   < 1e-10 relative on converged solutions. Requires H > 0 in the nonlinear case. */
+typedef enum ptd_arith { PTD_ARITH_EXACT = 0, PTD_ARITH_FAST = 1 } ptd_arith;
+
 /* Norm that stops the PT loop. The reference stops on RMS = norm(ResH)/sqrt(length(ResH))
  * (scripts/diff_3D/diff_3D_nonlin_perf.jl:106); PTD_NORM_MAX stops on max|ResH| instead. Both are
  * always computed. */
@@ -83,6 +92,8 @@ typedef struct ptd_params {
     int64_t itmax;      /* PT iteration cap per physical step (itMax) */
     int32_t nout;       /* residual check interval (nout) */
     int32_t stop_norm;  /* ptd_norm */
+    int32_t arith;      /* ptd_arith of the fused sweeps (the kernel-level entry points are always EXACT) */
+    int32_t reserved;   /* 0 */
     double dx, dy, dz;  /* cell sizes */
     double dt;          /* physical time step (theta_kin for the reaction scripts) */
     double vpdt;        /* Vpdtau = CFL*min(dx,dy,dz) */

@@ -17,6 +17,7 @@ LIN, STEP, NONLIN = 0, 1, 2
 PLAIN, PERF = 0, 1
 FP64, FP32, MIXED = 0, 1, 2
 NORM_RMS, NORM_MAX = 0, 1
+ARITH_EXACT, ARITH_FAST = 0, 1
 F_H, F_HOLD, F_D, F_QX, F_QY, F_QZ, F_QX2, F_QY2, F_QZ2, F_RESH, F_DTAU = range(11)
 
 STATUS_NAMES = {OK: "PTD_OK", ERR_ARG: "PTD_ERR_ARG", ERR_HIP: "PTD_ERR_HIP", ERR_NAN: "PTD_ERR_NAN",
@@ -46,6 +47,7 @@ class Params(ctypes.Structure):
         ("precision", ctypes.c_int32),
         ("nx", ctypes.c_int64), ("ny", ctypes.c_int64), ("nz", ctypes.c_int64),
         ("itmax", ctypes.c_int64), ("nout", ctypes.c_int32), ("stop_norm", ctypes.c_int32),
+        ("arith", ctypes.c_int32), ("reserved", ctypes.c_int32),
         ("dx", ctypes.c_double), ("dy", ctypes.c_double), ("dz", ctypes.c_double),
         ("dt", ctypes.c_double), ("vpdt", ctypes.c_double), ("max_l", ctypes.c_double),
         ("max_l2", ctypes.c_double), ("resc", ctypes.c_double), ("d0", ctypes.c_double),

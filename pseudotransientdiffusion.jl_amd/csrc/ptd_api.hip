@@ -42,6 +42,7 @@ int check_params(const ptd_params* p) {
     PTD_REQUIRE(p->precision == PTD_FP64 || p->precision == PTD_FP32, PTD_ERR_ARG,
                 "precision must be PTD_FP64 or PTD_FP32 for a context");
     PTD_REQUIRE(p->stop_norm == PTD_NORM_RMS || p->stop_norm == PTD_NORM_MAX, PTD_ERR_ARG, "stop_norm must be PTD_NORM_RMS or PTD_NORM_MAX");
+    PTD_REQUIRE(p->arith == PTD_ARITH_EXACT || p->arith == PTD_ARITH_FAST, PTD_ERR_ARG, "arith must be PTD_ARITH_EXACT or PTD_ARITH_FAST");
     PTD_REQUIRE(p->nx >= 3, PTD_ERR_ARG, "nx must be >= 3");
     PTD_REQUIRE(p->ndim < 2 || p->ny >= 3, PTD_ERR_ARG, "ny must be >= 3");
     PTD_REQUIRE(p->ndim < 3 || p->nz >= 3, PTD_ERR_ARG, "nz must be >= 3");

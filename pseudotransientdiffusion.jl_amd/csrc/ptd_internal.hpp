@@ -26,6 +26,13 @@ struct KP {
     T c_lv;               // max_l / Vpdtau      (first division of max_l / Vpdtau / Re * Resc)
     T c_vl;               // Vpdtau * max_l      (first product of Vpdtau * max_l / H3 / Re * Resc)
     T max_l2, resc, d0, theta_r, dtau_rho, re;
+    // constants of the FAST arithmetic (rearranged update rules)
+    T K;                  // max_l2 * _dt
+    T c1;                 // max_l / Vpdtau * Resc           (theta = c1 / Re)
+    T c2;                 // Vpdtau * max_l * Resc           (nonlinear dtau_rho = c2 / (H^3 Re))
+    T c3;                 // Vpdtau * max_l / Re             (step dtau_rho = c3 / maxloc(D))
+    T inv1p_theta;        // 1 / (1 + theta_r_dtau)          (lin/step)
+    T inv1p_dtau;         // 1 / (1 + _dt * dtau_rho)        (lin)
 };
 
 template <typename T>
@@ -57,6 +64,13 @@ KP<T> make_kp(const ptd_params& p) {
     k.theta_r = (T)p.theta_r;
     k.dtau_rho = (T)p.dtau_rho;
     k.re = (T)p.re;
+    const double _dt = 1.0 / p.dt;
+    k.K = (T)(p.max_l2 * _dt);
+    k.c1 = (T)(p.max_l / p.vpdt * p.resc);
+    k.c2 = (T)(p.vpdt * p.max_l * p.resc);
+    k.c3 = (T)(p.re > 0 ? p.vpdt * p.max_l / p.re : 0.0);
+    k.inv1p_theta = (T)(1.0 / (1.0 + p.theta_r));
+    k.inv1p_dtau = (T)(1.0 / (1.0 + _dt * p.dtau_rho));
     return k;
 }
 
@@ -167,6 +181,139 @@ __device__ __forceinline__ T residual(const KP<T>& p, T H, T Hold, T div) {
     if constexpr (FORM == PTD_PERF) return -p._dt * (H - Hold) - div;
     else return -(H - Hold) / p.dt - div;
 }
+
+// ---------------------------------------------------------------------------------------------
+// FAST arithmetic: Newton-refined hardware reciprocal / reciprocal square root
+// ---------------------------------------------------------------------------------------------
+
+// 1/x: v_rcp_f64 then two Newton steps (relative error below 2 ulp for normal x).
+__device__ __forceinline__ double rcp_nr(double x) {
+    double r = __builtin_amdgcn_rcp(x);
+    double e = fma(-x, r, 1.0);
+    r = fma(r, e, r);
+    e = fma(-x, r, 1.0);
+    return fma(r, e, r);
+}
+__device__ __forceinline__ float rcp_nr(float x) {
+    float r = __builtin_amdgcn_rcpf(x);
+    return fmaf(r, fmaf(-x, r, 1.0f), r);
+}
+
+// sqrt(x) for normal x > 0: x*rsq(x) with one coupled Newton step on (g, h) = (sqrt x, 1/(2 sqrt x))
+// and a final residual correction.
+__device__ __forceinline__ double sqrt_nr(double x) {
+    const double y = __builtin_amdgcn_rsq(x);
+    double g = x * y, h = 0.5 * y;
+    const double r = fma(-g, h, 0.5);
+    g = fma(g, r, g);
+    h = fma(h, r, h);
+    const double d = fma(-g, g, x);
+    return fma(d, h, g);
+}
+__device__ __forceinline__ float sqrt_nr(float x) { return __builtin_amdgcn_sqrtf(x); }
+
+template <typename T>
+__device__ __forceinline__ T fmaT(T a, T b, T c) { return fma(a, b, c); }
+
+// ---------------------------------------------------------------------------------------------
+// one face / one cell, EXACT (reference order) or FAST (rearranged)
+// ---------------------------------------------------------------------------------------------
+
+enum { kExact = PTD_ARITH_EXACT, kFast = PTD_ARITH_FAST };
+
+template <typename T, int DC, int FORM, int ARITH>
+struct Phys {
+    // New damped flux on the face (a | b) from the old flux q; g receives the residual flux
+    // (-coef*dH/h, the q2 of the reference) when wanted.
+    template <bool Q2>
+    __device__ static __forceinline__ T face(const KP<T>& p, T q, T Ha, T Hb, T Da, T Db, T h, T ih, T& q2) {
+        const T dH = Hb - Ha;
+        if constexpr (ARITH == kExact) {
+            const T coef = face_coef<T, DC, FORM>(p, Ha, Hb, Da, Db);
+            if constexpr (Q2) q2 = res_flux<T, FORM>(coef, dH, h, ih);
+            return damped_flux<T, FORM>(q, face_theta<T, DC, FORM>(p, coef), coef, dH, h, ih);
+        } else {
+            if constexpr (DC == PTD_NONLIN) {
+                // q' = (q*theta - g)/(1+theta), theta = c1/Re  ==>  q' = (q*c1 - g*Re)/(Re + c1)
+                const T hf = (T)0.5 * (Ha + Hb);
+                const T h3 = hf * hf * hf;
+                const T re = p.pi + sqrt_nr(fmaT(p.K, rcp_nr(h3), p.pi2));
+                const T g = h3 * (ih * dH);
+                if constexpr (Q2) q2 = -g;
+                return fmaT(q, p.c1, -(g * re)) * rcp_nr(re + p.c1);
+            } else {
+                T coef;
+                if constexpr (DC == PTD_LIN) coef = p.d0;
+                else coef = (Da + Db) * (T)0.5;
+                const T g = coef * (ih * dH);
+                if constexpr (Q2) q2 = -g;
+                return fmaT(q, p.theta_r, -g) * p.inv1p_theta;
+            }
+        }
+    }
+
+    // Residual flux only (compute_flux_res!).
+    __device__ static __forceinline__ T face_res(const KP<T>& p, T Ha, T Hb, T Da, T Db, T h, T ih) {
+        const T dH = Hb - Ha;
+        if constexpr (ARITH == kExact) {
+            return res_flux<T, FORM>(face_coef<T, DC, FORM>(p, Ha, Hb, Da, Db), dH, h, ih);
+        } else {
+            T coef;
+            if constexpr (DC == PTD_NONLIN) {
+                const T hf = (T)0.5 * (Ha + Hb);
+                coef = hf * hf * hf;
+            } else if constexpr (DC == PTD_LIN) {
+                coef = p.d0;
+            } else {
+                coef = (Da + Db) * (T)0.5;
+            }
+            return -(coef * (ih * dH));
+        }
+    }
+
+    template <int NDIM>
+    __device__ static __forceinline__ T div(const KP<T>& p, T ddx, T ddy, T ddz) {
+        if constexpr (ARITH == kExact) {
+            return divergence<T, FORM, NDIM>(p, ddx, ddy, ddz);
+        } else {
+            T s = p._dx * ddx;
+            if constexpr (NDIM >= 2) s = fmaT(p._dy, ddy, s);
+            if constexpr (NDIM >= 3) s = fmaT(p._dz, ddz, s);
+            return s;
+        }
+    }
+
+    // H update of one cell from the flux divergence; dmax = @maxloc(D) (step case only).
+    __device__ static __forceinline__ T cell(const KP<T>& p, T H, T Hold, T div, T dmax) {
+        if constexpr (ARITH == kExact) {
+            T dtr;
+            if constexpr (DC == PTD_LIN) dtr = p.dtau_rho;
+            else if constexpr (DC == PTD_STEP) dtr = step_dtau(p, dmax);
+            else dtr = nonlin_dtau<T, FORM>(p, H);
+            return h_update<T, FORM>(p, H, Hold, dtr, div);
+        } else {
+            const T src = fmaT(p._dt, Hold, -div);
+            if constexpr (DC == PTD_LIN) {
+                // (H + dtau*src)/(1 + _dt*dtau)
+                return fmaT(p.dtau_rho, src, H) * p.inv1p_dtau;
+            } else if constexpr (DC == PTD_STEP) {
+                // dtau = c3/m  ==>  (H*m + c3*src)/(m + _dt*c3)
+                return fmaT(H, dmax, p.c3 * src) * rcp_nr(fmaT(p.c3, p._dt, dmax));
+            } else {
+                // dtau = c2/(H^3 Re) = c2/P  ==>  (H*P + c2*src)/(P + _dt*c2)
+                const T h3 = H * H * H;
+                const T re = p.pi + sqrt_nr(fmaT(p.K, rcp_nr(h3), p.pi2));
+                const T P = h3 * re;
+                return fmaT(H, P, p.c2 * src) * rcp_nr(fmaT(p.c2, p._dt, P));
+            }
+        }
+    }
+
+    __device__ static __forceinline__ T resid(const KP<T>& p, T H, T Hold, T div) {
+        if constexpr (ARITH == kExact) return residual<T, FORM>(p, H, Hold, div);
+        else return fmaT(-p._dt, H - Hold, -div);
+    }
+};
 
 }  // namespace ptd
 

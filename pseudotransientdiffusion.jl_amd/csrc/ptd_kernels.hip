@@ -213,14 +213,37 @@ k_reduce_partials(const double2* __restrict__ partial, int64_t n, double* __rest
 // Updated cells: lanes 1..62, rows 1..R-2. z: the block marches k = k0..k1-1; the upper z face flux of
 // plane k is carried in a register as the lower z face of plane k+1. Loads for plane k+1 are issued
 // before plane k is computed. Each face and each cell is written by exactly one thread.
+// Addressing: the plane base of every array is block-uniform (scalar registers); each thread keeps a
+// 32-bit byte offset into the plane.
 // CHECK also evaluates the plain form's residual (residual fluxes from the pre-update H, transient
 // term from the updated H: scripts/diff_3D/diff_3D_nonlin_multixpu.jl:41-43,53) and reduces it.
-template <typename T, int DC, int FORM, int R, bool CHECK>
+
+template <typename T>
+__device__ __forceinline__ T ldp(const T* __restrict__ plane, uint32_t off, bool ok) {
+    return ok ? *reinterpret_cast<const T*>(reinterpret_cast<const char*>(plane) + off) : (T)0;
+}
+template <typename T>
+__device__ __forceinline__ void stp(T* __restrict__ plane, uint32_t off, T v) {
+    *reinterpret_cast<T*>(reinterpret_cast<char*>(plane) + off) = v;
+}
+
+// Operands of one plane p of the march (loaded two planes ahead).
+template <typename T, bool STEP>
+struct PlaneOps {
+    T Hn;      // H(p+1)
+    T Dn;      // D(p+1) (step case)
+    T Ho;      // Hold(p)
+    T qx, qy;  // old fluxes on this thread's lower x / y faces at plane p
+    T qz;      // old flux on the upper z face (p | p+1)
+};
+
+template <typename T, int DC, int FORM, int ARITH, int R, bool CHECK>
 __global__ void __launch_bounds__(64 * R)
 k_fused3d(const KP<T> p, const T* __restrict__ Hin, T* __restrict__ Hout, const T* __restrict__ Hold,
           const T* __restrict__ D, const T* __restrict__ qxi, const T* __restrict__ qyi,
           const T* __restrict__ qzi, T* __restrict__ qxo, T* __restrict__ qyo, T* __restrict__ qzo,
           double2* __restrict__ partial, int tz, int ntx, int nty, int ntiles, int per_xcd) {
+    using Ph = Phys<T, DC, FORM, ARITH>;
     constexpr bool STEP = DC == PTD_STEP;
     __shared__ T sH[R][64];
     __shared__ T sQ[R][64];
@@ -238,13 +261,12 @@ k_fused3d(const KP<T> p, const T* __restrict__ Hin, T* __restrict__ Hout, const 
     const int bz = rest / nty;
 
     const int lane = threadIdx.x, row = threadIdx.y;
-    const int64_t nx = p.nx, ny = p.ny, nz = p.nz;
-    const int64_t i = 62 * (int64_t)bx + lane;
-    const int64_t j = (int64_t)(R - 2) * by + row;
-    const int64_t k0 = 1 + (int64_t)tz * bz;
-    const int64_t k1 = (k0 + tz < nz - 1) ? k0 + tz : nz - 1;
+    const int nx = (int)p.nx, ny = (int)p.ny, nz = (int)p.nz;
+    const int i = 62 * bx + lane;
+    const int j = (R - 2) * by + row;
+    const int k0 = 1 + tz * bz;
+    const int k1 = (k0 + tz < nz - 1) ? k0 + tz : nz - 1;
 
-    const bool ld = (i <= nx - 1) && (j <= ny - 1);
     const bool xin = (i >= 1) && (i <= nx - 2);
     const bool yin = (j >= 1) && (j <= ny - 2);
     const bool rmid = (row >= 1) && (row <= R - 2);
@@ -255,124 +277,109 @@ k_fused3d(const KP<T> p, const T* __restrict__ Hin, T* __restrict__ Hout, const 
     const bool wx = fx && ((lane <= 62 && i <= nx - 2) || i == nx - 1);
     const bool wy = fy && ((row <= R - 2 && j <= ny - 2) || j == ny - 1);
 
-    const int64_t psH = nx * ny, offH = i + nx * j;
-    const int64_t psQx = (nx - 1) * (ny - 2), offQx = (i - 1) + (nx - 1) * (j - 1);
-    const int64_t psQy = (nx - 2) * (ny - 1), offQy = (i - 1) + (nx - 2) * (j - 1);
-    const int64_t psQz = (nx - 2) * (ny - 2), offQz = (i - 1) + (nx - 2) * (j - 1);
+    // Plane strides (elements) and per-thread byte offsets within a plane. Offsets are clamped into
+    // each array's index range so every load is in bounds; values of lanes outside the computed set
+    // are never stored and never reach a computed cell.
+    const size_t psH = (size_t)nx * ny;
+    const size_t psQx = (size_t)(nx - 1) * (ny - 2);
+    const size_t psQy = (size_t)(nx - 2) * (ny - 1);
+    const size_t psQz = (size_t)(nx - 2) * (ny - 2);
+    const int ic = min(i, nx - 1), jc = min(j, ny - 1);
+    const int ixq = max(1, min(i, nx - 1)), jxq = max(1, min(j, ny - 2));
+    const int iyq = max(1, min(i, nx - 2)), jyq = max(1, min(j, ny - 1));
+    const int jzq = max(1, min(j, ny - 2));
+    constexpr uint32_t es = (uint32_t)sizeof(T);
+    const uint32_t oH = (uint32_t)(ic + nx * jc) * es;
+    const uint32_t oQx = (uint32_t)((ixq - 1) + (nx - 1) * (jxq - 1)) * es;
+    const uint32_t oQy = (uint32_t)((iyq - 1) + (nx - 2) * (jyq - 1)) * es;
+    const uint32_t oQz = (uint32_t)((iyq - 1) + (nx - 2) * (jzq - 1)) * es;
+
+    auto load_plane = [&](int pl) {  // operands of plane pl (clamped to the chunk's last plane)
+        pl = pl < k1 ? pl : k1 - 1;
+        PlaneOps<T, STEP> o;
+        o.Hn = ldp(Hin + psH * (pl + 1), oH, true);
+        if constexpr (STEP) o.Dn = ldp(D + psH * (pl + 1), oH, true);
+        else o.Dn = 0;
+        o.Ho = ldp(Hold + psH * pl, oH, true);
+        o.qx = ldp(qxi + psQx * (pl - 1), oQx, true);
+        o.qy = ldp(qyi + psQy * (pl - 1), oQy, true);
+        o.qz = ldp(qzi + psQz * pl, oQz, true);
+        return o;
+    };
 
     // lower z face of the first plane
-    T Hc = load_or0(Hin, offH + psH * k0, ld);
+    T Hc = ldp(Hin + psH * k0, oH, true);
+    const T Hkm = ldp(Hin + psH * (k0 - 1), oH, true);
     T Dc = 0, Dkm = 0;
     if constexpr (STEP) {
-        Dc = load_or0(D, offH + psH * k0, ld);
-        Dkm = load_or0(D, offH + psH * (k0 - 1), upd);
+        Dc = ldp(D + psH * k0, oH, true);
+        Dkm = ldp(D + psH * (k0 - 1), oH, true);
     }
-    T qzL = 0, q2zL = 0;
-    if (upd) {
-        const T Hkm = Hin[offH + psH * (k0 - 1)];
-        const T coef = face_coef<T, DC, FORM>(p, Hkm, Hc, Dkm, Dc);
-        const T dH = Hc - Hkm;
-        const T th = face_theta<T, DC, FORM>(p, coef);
-        qzL = damped_flux<T, FORM>(qzi[offQz + psQz * (k0 - 1)], th, coef, dH, p.dz, p._dz);
-        if constexpr (CHECK) q2zL = res_flux<T, FORM>(coef, dH, p.dz, p._dz);
-        if (k0 == 1) qzo[offQz] = qzL;
-    }
+    T q2zL = 0;
+    T qzL = Ph::template face<CHECK>(p, ldp(qzi + psQz * (k0 - 1), oQz, true), Hkm, Hc, Dkm, Dc, p.dz, p._dz, q2zL);
+    if (upd && k0 == 1) stp(qzo, oQz, qzL);
 
-    // operands of plane k0
-    T nH = load_or0(Hin, offH + psH * (k0 + 1), ld);
-    T nD = 0;
-    if constexpr (STEP) nD = load_or0(D, offH + psH * (k0 + 1), ld);
-    T nHo = load_or0(Hold, offH + psH * k0, upd);
-    T nqx = load_or0(qxi, offQx + psQx * (k0 - 1), fx);
-    T nqy = load_or0(qyi, offQy + psQy * (k0 - 1), fy);
-    T nqz = load_or0(qzi, offQz + psQz * k0, upd);
+    PlaneOps<T, STEP> A = load_plane(k0);
+    PlaneOps<T, STEP> B = load_plane(k0 + 1);
 
     double acc = 0.0, amax = 0.0;
-    for (int64_t k = k0; k < k1; ++k) {
-        const T Hn = nH, Dn = nD, Ho = nHo, qx_old = nqx, qy_old = nqy, qz_old = nqz;
-        if (k + 1 < k1) {  // prefetch plane k+1 (k+2 <= nz-1 here)
-            nH = load_or0(Hin, offH + psH * (k + 2), ld);
-            if constexpr (STEP) nD = load_or0(D, offH + psH * (k + 2), ld);
-            nHo = load_or0(Hold, offH + psH * (k + 1), upd);
-            nqx = load_or0(qxi, offQx + psQx * k, fx);
-            nqy = load_or0(qyi, offQy + psQy * k, fy);
-            nqz = load_or0(qzi, offQz + psQz * (k + 1), upd);
-        }
+    for (int k = k0; k < k1; ++k) {
         sH[row][lane] = Hc;
         if constexpr (STEP) sD[row][lane] = Dc;
         __syncthreads();
 
         const T Hw = __shfl_up(Hc, 1);
-        const T Hs = row > 0 ? sH[row - 1][lane] : (T)0;
+        const T Hs = sH[row > 0 ? row - 1 : 0][lane];
         T Dw = 0, De = 0, Ds = 0, Dnn = 0;
         if constexpr (STEP) {
             Dw = __shfl_up(Dc, 1);
             De = __shfl_down(Dc, 1);
-            Ds = row > 0 ? sD[row - 1][lane] : (T)0;
-            Dnn = row < R - 1 ? sD[row + 1][lane] : (T)0;
+            Ds = sD[row > 0 ? row - 1 : 0][lane];
+            Dnn = sD[row < R - 1 ? row + 1 : R - 1][lane];
         }
-        T qxn = 0, qyn = 0, qzn = 0, q2x = 0, q2y = 0, q2z = 0;
-        if (fx) {
-            const T coef = face_coef<T, DC, FORM>(p, Hw, Hc, Dw, Dc);
-            const T dH = Hc - Hw;
-            qxn = damped_flux<T, FORM>(qx_old, face_theta<T, DC, FORM>(p, coef), coef, dH, p.dx, p._dx);
-            if constexpr (CHECK) q2x = res_flux<T, FORM>(coef, dH, p.dx, p._dx);
-        }
-        if (fy) {
-            const T coef = face_coef<T, DC, FORM>(p, Hs, Hc, Ds, Dc);
-            const T dH = Hc - Hs;
-            qyn = damped_flux<T, FORM>(qy_old, face_theta<T, DC, FORM>(p, coef), coef, dH, p.dy, p._dy);
-            if constexpr (CHECK) q2y = res_flux<T, FORM>(coef, dH, p.dy, p._dy);
-        }
-        if (upd) {
-            const T coef = face_coef<T, DC, FORM>(p, Hc, Hn, Dc, Dn);
-            const T dH = Hn - Hc;
-            qzn = damped_flux<T, FORM>(qz_old, face_theta<T, DC, FORM>(p, coef), coef, dH, p.dz, p._dz);
-            if constexpr (CHECK) q2z = res_flux<T, FORM>(coef, dH, p.dz, p._dz);
-        }
+        T q2x = 0, q2y = 0, q2z = 0;
+        const T qxn = Ph::template face<CHECK>(p, A.qx, Hw, Hc, Dw, Dc, p.dx, p._dx, q2x);
+        const T qyn = Ph::template face<CHECK>(p, A.qy, Hs, Hc, Ds, Dc, p.dy, p._dy, q2y);
+        const T qzn = Ph::template face<CHECK>(p, A.qz, Hc, A.Hn, Dc, A.Dn, p.dz, p._dz, q2z);
         sQ[row][lane] = qyn;
         if constexpr (CHECK) sQ2[row][lane] = q2y;
         __syncthreads();
 
         const T qxE = __shfl_down(qxn, 1);
-        const T qyN = row < R - 1 ? sQ[row + 1][lane] : (T)0;
-        T q2xE = 0, q2yN = 0;
+        const T qyN = sQ[row < R - 1 ? row + 1 : R - 1][lane];
+        T dmax = 0;
+        if constexpr (STEP) {
+            dmax = fmax(fmax(Dw, De), Dc);
+            dmax = fmax(dmax, fmax(Ds, Dnn));
+            dmax = fmax(dmax, fmax(Dkm, A.Dn));
+        }
+        const T dv = Ph::template div<3>(p, qxE - qxn, qyN - qyn, qzn - qzL);
+        const T Hnew = Ph::cell(p, Hc, A.Ho, dv, dmax);
         if constexpr (CHECK) {
-            q2xE = __shfl_down(q2x, 1);
-            q2yN = row < R - 1 ? sQ2[row + 1][lane] : (T)0;
+            const T q2xE = __shfl_down(q2x, 1);
+            const T q2yN = sQ2[row < R - 1 ? row + 1 : R - 1][lane];
+            const T dv2 = Ph::template div<3>(p, q2xE - q2x, q2yN - q2y, q2z - q2zL);
+            const double r = upd ? (double)Ph::resid(p, Hnew, A.Ho, dv2) : 0.0;
+            acc += r * r;
+            amax = nanmax(fabs(r), amax);
         }
+        // stores of plane k, then the operands of plane k+2 (so waiting for plane k+1's operands
+        // never waits for these stores)
         if (upd) {
-            T dtr;
-            if constexpr (DC == PTD_LIN) {
-                dtr = p.dtau_rho;
-            } else if constexpr (STEP) {
-                T m = fmax(fmax(Dw, De), Dc);
-                m = fmax(m, fmax(Ds, Dnn));
-                m = fmax(m, fmax(Dkm, Dn));
-                dtr = step_dtau(p, m);
-            } else {
-                dtr = nonlin_dtau<T, FORM>(p, Hc);
-            }
-            const T div = divergence<T, FORM, 3>(p, qxE - qxn, qyN - qyn, qzn - qzL);
-            const T Hnew = h_update<T, FORM>(p, Hc, Ho, dtr, div);
-            Hout[offH + psH * k] = Hnew;
-            qzo[offQz + psQz * k] = qzn;
-            if constexpr (CHECK) {
-                const T div2 = divergence<T, FORM, 3>(p, q2xE - q2x, q2yN - q2y, q2z - q2zL);
-                const double r = (double)residual<T, FORM>(p, Hnew, Ho, div2);
-                acc += r * r;
-                amax = nanmax(fabs(r), amax);
-            }
+            stp(Hout + psH * k, oH, Hnew);
+            stp(qzo + psQz * k, oQz, qzn);
         }
-        if (wx) qxo[offQx + psQx * (k - 1)] = qxn;
-        if (wy) qyo[offQy + psQy * (k - 1)] = qyn;
+        if (wx) stp(qxo + psQx * (k - 1), oQx, qxn);
+        if (wy) stp(qyo + psQy * (k - 1), oQy, qyn);
         qzL = qzn;
         q2zL = q2z;
-        Hc = Hn;
+        Hc = A.Hn;
         if constexpr (STEP) {
             Dkm = Dc;
-            Dc = Dn;
+            Dc = A.Dn;
         }
+        A = B;
+        B = load_plane(k + 2);
     }
     if constexpr (CHECK) block_sum_max(acc, amax, row * 64 + lane, 64 * R, &partial[tile]);
 }
@@ -460,8 +467,8 @@ static int fused_tz(const ptd_ctx* c) {
     return (int)std::min<int64_t>(tz, mz);
 }
 
-template <typename T, int DC, int FORM>
-static hipError_t fused_t(ptd_ctx* c, bool check) {
+template <typename T, int DC, int FORM, int ARITH>
+static hipError_t fused_ta(ptd_ctx* c, bool check) {
     const KP<T> kp = make_kp<T>(c->p);
     const int tz = fused_tz(c);
     const int ntx = (int)((c->p.nx - 2 + 61) / 62);
@@ -476,11 +483,11 @@ static hipError_t fused_t(ptd_ctx* c, bool check) {
     const T* Ho = (const T*)c->Hold;
     const T* D = (const T*)c->D;
     if (check)
-        hipLaunchKernelGGL((k_fused3d<T, DC, FORM, kRows, true>), g, bl, 0, c->stream, kp, Hin, Hout, Ho, D,
+        hipLaunchKernelGGL((k_fused3d<T, DC, FORM, ARITH, kRows, true>), g, bl, 0, c->stream, kp, Hin, Hout, Ho, D,
                            (const T*)c->q[0][q], (const T*)c->q[1][q], (const T*)c->q[2][q], (T*)c->q[0][1 - q],
                            (T*)c->q[1][1 - q], (T*)c->q[2][1 - q], (double2*)c->partial, tz, ntx, nty, ntiles, per);
     else
-        hipLaunchKernelGGL((k_fused3d<T, DC, FORM, kRows, false>), g, bl, 0, c->stream, kp, Hin, Hout, Ho, D,
+        hipLaunchKernelGGL((k_fused3d<T, DC, FORM, ARITH, kRows, false>), g, bl, 0, c->stream, kp, Hin, Hout, Ho, D,
                            (const T*)c->q[0][q], (const T*)c->q[1][q], (const T*)c->q[2][q], (T*)c->q[0][1 - q],
                            (T*)c->q[1][1 - q], (T*)c->q[2][1 - q], (double2*)c->partial, tz, ntx, nty, ntiles, per);
     hipError_t e = hipGetLastError();
@@ -491,6 +498,12 @@ static hipError_t fused_t(ptd_ctx* c, bool check) {
         hipLaunchKernelGGL(k_reduce_partials, dim3(1), dim3(kBlock), 0, c->stream, (const double2*)c->partial,
                            (int64_t)ntiles, c->dred);
     return hipGetLastError();
+}
+
+template <typename T, int DC, int FORM>
+static hipError_t fused_t(ptd_ctx* c, bool check) {
+    if (c->p.arith == PTD_ARITH_FAST) return fused_ta<T, DC, FORM, kFast>(c, check);
+    return fused_ta<T, DC, FORM, kExact>(c, check);
 }
 
 // dispatch on (precision, dcase, form)

@@ -80,7 +80,7 @@ def linrange(a, b, n):
 
 
 def make_params(spec, nx, ny=1, nz=1, precision=_abi.FP64, form=None, stop_norm=_abi.NORM_RMS,
-                norm_len=0.0):
+                norm_len=0.0, arith=_abi.ARITH_EXACT):
     """ptd_params from a script's constants, derived as in its "Derived numerics" block
     (e.g. scripts/diff_3D/diff_3D_nonlin_perf.jl:73-78, scripts/diff_2D_lin.jl:48-53)."""
     nd = spec["ndim"]
@@ -107,6 +107,7 @@ def make_params(spec, nx, ny=1, nz=1, precision=_abi.FP64, form=None, stop_norm=
     p.itmax = int(spec["itMax"])
     p.nout = int(spec["nout"])
     p.stop_norm = stop_norm
+    p.arith = arith
     p.dx, p.dy, p.dz, p.dt = dx, dy, dz, dt
     p.vpdt, p.max_l, p.max_l2 = vpdt, max_l, max_l2
     p.resc, p.d0, p.re = spec["Resc"], d0, re
@@ -220,8 +221,10 @@ def a_eff_gb(ndim, n, esz=8):
 
 
 def run_script(name, nx, ny=None, nz=None, precision=_abi.FP64, device=0, ic=None, amp=2.0, bval=1.0,
-               dstep=None, **over):
-    """include("scripts/<name>.jl") with USE_RETURN=true, on the GPU. Returns a ScriptResult."""
+               dstep=None, H0=None, D=None, **over):
+    """include("scripts/<name>.jl") with USE_RETURN=true, on the GPU. Returns a ScriptResult.
+
+    H0 / D, when given, replace the script's initial condition / step field (same shape)."""
     spec = dict(SCRIPTS[name])
     spec.update(over)
     nd = spec["ndim"]
@@ -229,10 +232,12 @@ def run_script(name, nx, ny=None, nz=None, precision=_abi.FP64, device=0, ic=Non
     nz = nx if nz is None else nz
     n = [nx, ny, nz][:nd]
     params = make_params(spec, nx, ny, nz, precision=precision, norm_len=spec.get("norm_len", 0.0),
-                         stop_norm=spec.get("stop_norm", _abi.NORM_RMS))
+                         stop_norm=spec.get("stop_norm", _abi.NORM_RMS),
+                         arith=spec.get("arith", _abi.ARITH_EXACT))
     dxs = [spec[k] / n[d] for d, k in enumerate(("lx", "ly", "lz")[:nd])]
     coords = tuple(linrange(dxs[d] / 2, spec[("lx", "ly", "lz")[d]] - dxs[d] / 2, n[d]) for d in range(nd))
-    H0 = initial_condition(spec, nx, ny, nz, kind=ic, amp=amp, bval=bval)
+    if H0 is None:
+        H0 = initial_condition(spec, nx, ny, nz, kind=ic, amp=amp, bval=bval)
     nt = 1 if spec.get("react") else int(math.ceil(spec["ttot"] / spec["dt"]))
     with Context(params, device) as ctx:
         if spec.get("react"):
@@ -242,7 +247,7 @@ def run_script(name, nx, ny=None, nz=None, precision=_abi.FP64, device=0, ic=Non
             ctx.set("H", H0)
             ctx.set("Hold", H0)
         if spec["dcase"] == STEP:
-            ctx.set("D", step_field(spec, nx, ny, nz, kind=dstep))
+            ctx.set("D", step_field(spec, nx, ny, nz, kind=dstep) if D is None else D)
             ctx.compute_iter_params()
         iters = np.zeros(nt, dtype=np.int64)
         errs = np.zeros(nt)

@@ -31,12 +31,13 @@ def test_abi_version(pkg):
 
 
 def test_struct_layout_matches_header(pkg):
-    # ptd_params: 4 int32, 4 int64, 2 int32, 14 doubles
-    assert ctypes.sizeof(pkg.Params) == 4 * 4 + 4 * 8 + 2 * 4 + 14 * 8
+    # ptd_params: 4 int32, 4 int64, 4 int32, 14 doubles
+    assert ctypes.sizeof(pkg.Params) == 4 * 4 + 4 * 8 + 4 * 4 + 14 * 8
 
 
 @pytest.mark.parametrize("field,value", [("ndim", 4), ("dcase", 7), ("form", 3), ("precision", 9),
-                                         ("nx", 2), ("nout", 0), ("dx", 0.0), ("itmax", -1)])
+                                         ("nx", 2), ("nout", 0), ("dx", 0.0), ("itmax", -1),
+                                         ("arith", 5)])
 def test_invalid_params_rejected_before_device(field, value, pkg):
     from importlib import import_module
     abi = import_module("ptd_amd._abi")
