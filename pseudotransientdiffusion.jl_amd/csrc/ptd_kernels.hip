@@ -456,15 +456,14 @@ static hipError_t residual_t(ptd_ctx* c, bool write_resh) {
 
 static constexpr int kRows = 16;  // rows (waves) per fused 3D block
 
+// Planes per z chunk. Measured at 511^3 FP64 (FAST): 8 -> 2.20 ms, 16 -> 2.17, 32 -> 2.21, 64 -> 2.30,
+// 128 -> 2.45, 509 -> 3.01 ms per sweep: short chunks give many more blocks than CUs (small tail) and
+// the per-chunk prologue (one extra z face) stays cheap.
 static int fused_tz(const ptd_ctx* c) {
     const char* env = getenv("PTD_TZ");
     if (env && atoi(env) > 0) return atoi(env);
-    const int64_t ntx = (c->p.nx - 2 + 61) / 62, nty = (c->p.ny - 2 + kRows - 3) / (kRows - 2);
     const int64_t mz = c->p.nz - 2;
-    // enough z chunks for ~4 blocks per CU on 256 CUs, chunks of at least 16 planes
-    int64_t want = std::max<int64_t>(1, (1024 + ntx * nty - 1) / (ntx * nty));
-    int64_t tz = std::max<int64_t>(16, (mz + want - 1) / want);
-    return (int)std::min<int64_t>(tz, mz);
+    return (int)std::min<int64_t>(16, mz);
 }
 
 template <typename T, int DC, int FORM, int ARITH>
