@@ -4,6 +4,7 @@
 
 #include <cmath>
 #include <cstdio>
+#include <cstdlib>
 #include <cstring>
 #include <string>
 #include <vector>
@@ -192,6 +193,32 @@ int one_iteration(ptd_ctx* c, bool check) {
     if (check) {
         if (!plain) PTD_HIP(ptd::launch_flux(c, 2));
         PTD_HIP(ptd::launch_residual(c, false));
+    }
+    return PTD_OK;
+}
+
+// n PT iterations; with `check_last` the last one also leaves the residual reduction in dred. The 3D
+// path runs pairs of iterations through the two-level sweep (one HBM pass per pair) and a single
+// sweep for an odd remainder; PTD_T2=0 in the environment forces single sweeps.
+int run_iterations(ptd_ctx* c, int64_t n, bool check_last) {
+    const char* t2e = getenv("PTD_T2");
+    const bool t2_env = !(t2e && t2e[0] == '0');
+    const bool plain = c->p.form == PTD_PLAIN;
+    const bool t2 = t2_env && ptd::fused_available(c);
+    while (n > 0) {
+        if (t2 && n >= 2) {
+            const bool chk = check_last && n == 2;
+            PTD_HIP(ptd::launch_fused2(c, chk && plain));
+            if (chk && !plain) {
+                PTD_HIP(ptd::launch_flux(c, 2));
+                PTD_HIP(ptd::launch_residual(c, false));
+            }
+            n -= 2;
+        } else {
+            const int st = one_iteration(c, check_last && n == 1);
+            if (st != PTD_OK) return st;
+            n -= 1;
+        }
     }
     return PTD_OK;
 }
@@ -395,10 +422,8 @@ int32_t ptd_iterate(ptd_ctx* ctx, int64_t niter) {
     int st = device_guard(ctx);
     if (st != PTD_OK) return st;
     PTD_REQUIRE(niter >= 0, PTD_ERR_ARG, "niter must be >= 0");
-    for (int64_t it = 0; it < niter; ++it) {
-        st = one_iteration(ctx, false);
-        if (st != PTD_OK) return st;
-    }
+    st = run_iterations(ctx, niter, false);
+    if (st != PTD_OK) return st;
     PTD_HIP(hipStreamSynchronize(ctx->stream));
     return PTD_OK;
 }
@@ -408,10 +433,8 @@ int32_t ptd_iterate_timed(ptd_ctx* ctx, int64_t niter, double* ms) {
     if (st != PTD_OK) return st;
     PTD_REQUIRE(niter >= 0, PTD_ERR_ARG, "niter must be >= 0");
     PTD_HIP(hipEventRecord(ctx->ev0, ctx->stream));
-    for (int64_t it = 0; it < niter; ++it) {
-        st = one_iteration(ctx, false);
-        if (st != PTD_OK) return st;
-    }
+    st = run_iterations(ctx, niter, false);
+    if (st != PTD_OK) return st;
     PTD_HIP(hipEventRecord(ctx->ev1, ctx->stream));
     PTD_HIP(hipEventSynchronize(ctx->ev1));
     float f = 0.0f;
@@ -426,12 +449,15 @@ int32_t ptd_pt_solve(ptd_ctx* ctx, int64_t* iters, double* err_rms, double* err_
     const ptd_params& p = ctx->p;
     int64_t iter = 0;
     double err = 2.0 * p.tol, rms = err, mx = err;
-    // while err > tol && iter < itMax (scripts/diff_3D/diff_3D_nonlin_multixpu.jl:104-115)
+    // while err > tol && iter < itMax (scripts/diff_3D/diff_3D_nonlin_multixpu.jl:104-115). err only
+    // changes at checks, so the iterations up to the next check (or itMax) run as one block.
     while (err > p.tol && iter < p.itmax) {
-        const bool check = ((iter + 1) % p.nout) == 0;
-        st = one_iteration(ctx, check);
+        const int64_t next_check = (iter / p.nout + 1) * (int64_t)p.nout;
+        const int64_t stop = next_check < p.itmax ? next_check : p.itmax;
+        const bool check = (stop % p.nout) == 0;
+        st = run_iterations(ctx, stop - iter, check);
         if (st != PTD_OK) return st;
-        iter += 1;
+        iter = stop;
         if (check) {
             st = fetch_errors(ctx, &rms, &mx);
             if (st != PTD_OK) return st;

@@ -351,6 +351,7 @@ hipError_t launch_update(ptd_ctx* c);
 hipError_t launch_iter_params(ptd_ctx* c);
 hipError_t launch_residual(ptd_ctx* c, bool write_resh);  // partial sums -> dred
 hipError_t launch_fused(ptd_ctx* c, bool check);          // one fused PT iteration (3D)
+hipError_t launch_fused2(ptd_ctx* c, bool check);         // two fused PT iterations in one pass (3D)
 bool fused_available(const ptd_ctx* c);
 
 }  // namespace ptd

@@ -385,6 +385,224 @@ k_fused3d(const KP<T> p, const T* __restrict__ Hin, T* __restrict__ Hout, const 
 }
 
 // ---------------------------------------------------------------------------------------------
+// fused 3D sweep over two PT iterations (temporal blocking)
+// ---------------------------------------------------------------------------------------------
+//
+// Same z-marching block as k_fused3d, but every plane goes through two PT iterations before it is
+// written: level 1 (iteration n+1) runs one plane ahead of level 2 (iteration n+2). Level-1 values
+// (H1 and the level-1 fluxes) live only in registers and LDS, so HBM sees one read of the iterate-n
+// fields and one write of the iterate-(n+2) fields for two iterations. The tile halo shrinks by one
+// cell per level: lane L holds i = 60*bx - 1 + L and row r holds j = (R-4)*by - 1 + r; level 1 updates
+// lanes 1..62 / rows 1..R-2, level 2 updates lanes 2..61 / rows 2..R-3. Per plane step k: level 1 does
+// plane k, level 2 does plane k-1 (its upper z face needs H1 at plane k). Level 1 also runs the planes
+// k0-1 and k1 around the chunk. Cells on the domain boundary keep H (Dirichlet) at both levels.
+// Each level evaluates exactly the expressions of the one-iteration sweep, so two levels here give the
+// same bits as two k_fused3d launches.
+
+template <typename T, bool STEP>
+struct PlaneOps2 {
+    T Hn;      // H(k+1)
+    T Dn;      // D(k+1) (step)
+    T Ho;      // Hold(k)
+    T qx, qy;  // old fluxes on the lower x / y faces at plane k
+    T qz;      // old flux on the upper z face (k | k+1)
+};
+
+template <typename T, int DC, int FORM, int ARITH, int R, bool CHECK>
+__global__ void __launch_bounds__(64 * R)
+k_fused3d_t2(const KP<T> p, const T* __restrict__ Hin, T* __restrict__ Hout, const T* __restrict__ Hold,
+             const T* __restrict__ D, const T* __restrict__ qxi, const T* __restrict__ qyi,
+             const T* __restrict__ qzi, T* __restrict__ qxo, T* __restrict__ qyo, T* __restrict__ qzo,
+             double2* __restrict__ partial, int tz, int ntx, int nty, int ntiles, int per_xcd) {
+    using Ph = Phys<T, DC, FORM, ARITH>;
+    constexpr bool STEP = DC == PTD_STEP;
+    __shared__ T sH[R][64];                      // H at plane k (level-1 y neighbours)
+    __shared__ T sH1[R][64];                     // H1 at plane k-1 (level-2 y neighbours)
+    __shared__ T sQ1[R][64];                     // level-1 y flux at plane k
+    __shared__ T sQ2[R][64];                     // level-2 y flux at plane k-1
+    __shared__ T sD[STEP ? R : 1][64];           // D at plane k
+    __shared__ T sDp[STEP ? R : 1][64];          // D at plane k-1
+    __shared__ T sR2[CHECK ? R : 1][64];         // level-2 residual y flux
+
+    const int b = blockIdx.x;
+    const int tile = (b & 7) * per_xcd + (b >> 3);
+    if (tile >= ntiles) return;
+    const int bx = tile % ntx;
+    const int rest = tile / ntx;
+    const int by = rest % nty;
+    const int bz = rest / nty;
+
+    const int lane = threadIdx.x, row = threadIdx.y;
+    const int nx = (int)p.nx, ny = (int)p.ny, nz = (int)p.nz;
+    const int i = 60 * bx - 1 + lane;
+    const int j = (R - 4) * by - 1 + row;
+    const int k0 = 1 + tz * bz;
+    const int k1 = (k0 + tz < nz - 1) ? k0 + tz : nz - 1;  // level 2 updates planes k0..k1-1
+    const int ka = k0 - 1;                                  // level 1 runs planes ka..k1
+
+    const bool xin = (i >= 1) && (i <= nx - 2);
+    const bool yin = (j >= 1) && (j <= ny - 2);
+    const bool u1 = (lane >= 1) && (lane <= 62) && (row >= 1) && (row <= R - 2) && xin && yin;
+    const bool r2mid = (row >= 2) && (row <= R - 3);
+    const bool l2mid = (lane >= 2) && (lane <= 61);
+    const bool u2 = l2mid && r2mid && xin && yin;
+    const bool f2x = (lane >= 2) && (lane <= 62) && (i <= nx - 1) && r2mid && yin;
+    const bool f2y = (row >= 2) && (row <= R - 2) && (j <= ny - 1) && l2mid && xin;
+    const bool w2x = f2x && ((lane <= 61 && i <= nx - 2) || i == nx - 1);
+    const bool w2y = f2y && ((row <= R - 3 && j <= ny - 2) || j == ny - 1);
+
+    // Byte offsets, clamped into each array's range (every load is in bounds). Values of lanes outside
+    // a level's computed set are never stored and never reach a computed cell.
+    const size_t psH = (size_t)nx * ny;
+    const size_t psQx = (size_t)(nx - 1) * (ny - 2);
+    const size_t psQy = (size_t)(nx - 2) * (ny - 1);
+    const size_t psQz = (size_t)(nx - 2) * (ny - 2);
+    const int ic = max(0, min(i, nx - 1)), jc = max(0, min(j, ny - 1));
+    const int ixq = max(1, min(i, nx - 1)), jxq = max(1, min(j, ny - 2));
+    const int iyq = max(1, min(i, nx - 2)), jyq = max(1, min(j, ny - 1));
+    const int jzq = max(1, min(j, ny - 2));
+    constexpr uint32_t es = (uint32_t)sizeof(T);
+    const uint32_t oH = (uint32_t)(ic + nx * jc) * es;
+    const uint32_t oQx = (uint32_t)((ixq - 1) + (nx - 1) * (jxq - 1)) * es;
+    const uint32_t oQy = (uint32_t)((iyq - 1) + (nx - 2) * (jyq - 1)) * es;
+    const uint32_t oQz = (uint32_t)((iyq - 1) + (nx - 2) * (jzq - 1)) * es;
+
+    auto load_plane = [&](int pl) {  // level-1 operands of plane pl, pl in [0, nz-1]
+        PlaneOps2<T, STEP> o;
+        const int ph = pl + 1 < nz ? pl + 1 : nz - 1;
+        const int pq = pl - 1 < 0 ? 0 : (pl - 1 > nz - 3 ? nz - 3 : pl - 1);
+        const int pz = pl > nz - 2 ? nz - 2 : pl;
+        o.Hn = ldp(Hin + psH * ph, oH, true);
+        if constexpr (STEP) o.Dn = ldp(D + psH * ph, oH, true);
+        else o.Dn = 0;
+        o.Ho = ldp(Hold + psH * pl, oH, true);
+        o.qx = ldp(qxi + psQx * pq, oQx, true);
+        o.qy = ldp(qyi + psQy * pq, oQy, true);
+        o.qz = ldp(qzi + psQz * pz, oQz, true);
+        return o;
+    };
+
+    // level-1 lower z face of plane ka (face (ka-1 | ka)); needed only when plane ka is inner
+    T Hc = ldp(Hin + psH * ka, oH, true);
+    const int kb = ka > 0 ? ka - 1 : 0;
+    T Dc = 0, Dm1 = 0, Dm2 = 0;  // D at planes k, k-1, k-2
+    if constexpr (STEP) {
+        Dc = ldp(D + psH * ka, oH, true);
+        Dm1 = ldp(D + psH * kb, oH, true);
+    }
+    T unused = 0;
+    T q1zL = Ph::template face<false>(p, ldp(qzi + psQz * kb, oQz, true), ldp(Hin + psH * kb, oH, true), Hc,
+                                      Dm1, Dc, p.dz, p._dz, unused);
+
+    PlaneOps2<T, STEP> A = load_plane(ka);
+    PlaneOps2<T, STEP> B = load_plane(ka + 1 <= nz - 1 ? ka + 1 : nz - 1);
+
+    T H1p = 0;             // H1 at plane k-1
+    T Hop = 0;             // Hold at plane k-1
+    T q1xp = 0, q1yp = 0;  // level-1 fluxes on the lower x / y faces at plane k-1
+    T q2zL = 0, r2zL = 0;  // level-2 flux (and residual flux) on the lower z face of plane k-1
+    double acc = 0.0, amax = 0.0;
+
+    for (int k = ka; k <= k1; ++k) {
+        // phase A: planes into LDS
+        sH[row][lane] = Hc;
+        sH1[row][lane] = H1p;
+        if constexpr (STEP) {
+            sD[row][lane] = Dc;
+            sDp[row][lane] = Dm1;
+        }
+        __syncthreads();
+
+        // phase B: level-1 faces of plane k, level-2 x / y faces of plane k-1
+        const int rs = row > 0 ? row - 1 : 0, rn = row < R - 1 ? row + 1 : R - 1;
+        const T Hw = __shfl_up(Hc, 1);
+        const T Hs = sH[rs][lane];
+        T Dw = 0, De = 0, Ds = 0, Dnn = 0, Dpw = 0, Dpe = 0, Dps = 0, Dpn = 0;
+        if constexpr (STEP) {
+            Dw = __shfl_up(Dc, 1);
+            De = __shfl_down(Dc, 1);
+            Ds = sD[rs][lane];
+            Dnn = sD[rn][lane];
+            Dpw = __shfl_up(Dm1, 1);
+            Dpe = __shfl_down(Dm1, 1);
+            Dps = sDp[rs][lane];
+            Dpn = sDp[rn][lane];
+        }
+        const T q1x = Ph::template face<false>(p, A.qx, Hw, Hc, Dw, Dc, p.dx, p._dx, unused);
+        const T q1y = Ph::template face<false>(p, A.qy, Hs, Hc, Ds, Dc, p.dy, p._dy, unused);
+        const T q1zU = Ph::template face<false>(p, A.qz, Hc, A.Hn, Dc, A.Dn, p.dz, p._dz, unused);
+        const T H1w = __shfl_up(H1p, 1);
+        const T H1s = sH1[rs][lane];
+        T r2x = 0, r2y = 0;
+        const T q2x = Ph::template face<CHECK>(p, q1xp, H1w, H1p, Dpw, Dm1, p.dx, p._dx, r2x);
+        const T q2y = Ph::template face<CHECK>(p, q1yp, H1s, H1p, Dps, Dm1, p.dy, p._dy, r2y);
+        sQ1[row][lane] = q1y;
+        sQ2[row][lane] = q2y;
+        if constexpr (CHECK) sR2[row][lane] = r2y;
+        __syncthreads();
+
+        // phase C: level-1 update of plane k; level-2 z face (k-1 | k) and update of plane k-1
+        const T q1xE = __shfl_down(q1x, 1);
+        const T q1yN = sQ1[rn][lane];
+        T dmax1 = 0, dmax2 = 0;
+        if constexpr (STEP) {
+            dmax1 = fmax(fmax(Dw, De), Dc);
+            dmax1 = fmax(dmax1, fmax(Ds, Dnn));
+            dmax1 = fmax(dmax1, fmax(Dm1, A.Dn));
+            dmax2 = fmax(fmax(Dpw, Dpe), Dm1);
+            dmax2 = fmax(dmax2, fmax(Dps, Dpn));
+            dmax2 = fmax(dmax2, fmax(Dm2, Dc));
+        }
+        const T dv1 = Ph::template div<3>(p, q1xE - q1x, q1yN - q1y, q1zU - q1zL);
+        const T H1c = Ph::cell(p, Hc, A.Ho, dv1, dmax1);
+        const T H1 = (u1 && k >= 1 && k <= nz - 2) ? H1c : Hc;  // boundary cells keep H
+
+        T r2z = 0;
+        const T q2zU = Ph::template face<CHECK>(p, q1zL, H1p, H1, Dm1, Dc, p.dz, p._dz, r2z);
+        if (k > k0) {
+            const int m = k - 1;
+            const T q2xE = __shfl_down(q2x, 1);
+            const T q2yN = sQ2[rn][lane];
+            const T dv2 = Ph::template div<3>(p, q2xE - q2x, q2yN - q2y, q2zU - q2zL);
+            const T H2 = Ph::cell(p, H1p, Hop, dv2, dmax2);
+            if constexpr (CHECK) {
+                const T r2xE = __shfl_down(r2x, 1);
+                const T r2yN = sR2[rn][lane];
+                const T dvr = Ph::template div<3>(p, r2xE - r2x, r2yN - r2y, r2z - r2zL);
+                const double rr = u2 ? (double)Ph::resid(p, H2, Hop, dvr) : 0.0;
+                acc += rr * rr;
+                amax = nanmax(fabs(rr), amax);
+            }
+            if (u2) {
+                stp(Hout + psH * m, oH, H2);
+                stp(qzo + psQz * m, oQz, q2zU);
+            }
+            if (w2x) stp(qxo + psQx * (m - 1), oQx, q2x);
+            if (w2y) stp(qyo + psQy * (m - 1), oQy, q2y);
+        } else if (k == k0 && k0 == 1) {
+            if (u2) stp(qzo, oQz, q2zU);  // face (0 | 1): lower face of the first inner plane
+        }
+        // carry to plane k+1
+        q2zL = q2zU;
+        r2zL = r2z;
+        q1zL = q1zU;
+        H1p = H1;
+        Hop = A.Ho;
+        q1xp = q1x;
+        q1yp = q1y;
+        if constexpr (STEP) {
+            Dm2 = Dm1;
+            Dm1 = Dc;
+            Dc = A.Dn;
+        }
+        Hc = A.Hn;
+        A = B;
+        B = load_plane(k + 2 <= nz - 1 ? k + 2 : nz - 1);
+    }
+    if constexpr (CHECK) block_sum_max(acc, amax, row * 64 + lane, 64 * R, &partial[tile]);
+}
+
+// ---------------------------------------------------------------------------------------------
 // launchers
 // ---------------------------------------------------------------------------------------------
 
@@ -499,6 +717,45 @@ static hipError_t fused_ta(ptd_ctx* c, bool check) {
     return hipGetLastError();
 }
 
+template <typename T, int DC, int FORM, int ARITH>
+static hipError_t fused2_ta(ptd_ctx* c, bool check) {
+    const KP<T> kp = make_kp<T>(c->p);
+    const int tz = fused_tz(c);
+    const int ntx = (int)((c->p.nx - 2 + 59) / 60);
+    const int nty = (int)((c->p.ny - 2 + kRows - 5) / (kRows - 4));
+    const int nbz = (int)((c->p.nz - 2 + tz - 1) / tz);
+    const int ntiles = ntx * nty * nbz;
+    const int per = (ntiles + 7) / 8;
+    const dim3 g(8 * per), bl(64, kRows);
+    const int h = c->hcur, q = c->qcur;
+    const T* Hin = (const T*)c->H[h];
+    T* Hout = (T*)c->H[1 - h];
+    const T* Ho = (const T*)c->Hold;
+    const T* D = (const T*)c->D;
+    if (check)
+        hipLaunchKernelGGL((k_fused3d_t2<T, DC, FORM, ARITH, kRows, true>), g, bl, 0, c->stream, kp, Hin, Hout, Ho, D,
+                           (const T*)c->q[0][q], (const T*)c->q[1][q], (const T*)c->q[2][q], (T*)c->q[0][1 - q],
+                           (T*)c->q[1][1 - q], (T*)c->q[2][1 - q], (double2*)c->partial, tz, ntx, nty, ntiles, per);
+    else
+        hipLaunchKernelGGL((k_fused3d_t2<T, DC, FORM, ARITH, kRows, false>), g, bl, 0, c->stream, kp, Hin, Hout, Ho, D,
+                           (const T*)c->q[0][q], (const T*)c->q[1][q], (const T*)c->q[2][q], (T*)c->q[0][1 - q],
+                           (T*)c->q[1][1 - q], (T*)c->q[2][1 - q], (double2*)c->partial, tz, ntx, nty, ntiles, per);
+    hipError_t e = hipGetLastError();
+    if (e != hipSuccess) return e;
+    c->hcur = 1 - h;
+    c->qcur = 1 - q;
+    if (check)
+        hipLaunchKernelGGL(k_reduce_partials, dim3(1), dim3(kBlock), 0, c->stream, (const double2*)c->partial,
+                           (int64_t)ntiles, c->dred);
+    return hipGetLastError();
+}
+
+template <typename T, int DC, int FORM>
+static hipError_t fused2_t(ptd_ctx* c, bool check) {
+    if (c->p.arith == PTD_ARITH_FAST) return fused2_ta<T, DC, FORM, kFast>(c, check);
+    return fused2_ta<T, DC, FORM, kExact>(c, check);
+}
+
 template <typename T, int DC, int FORM>
 static hipError_t fused_t(ptd_ctx* c, bool check) {
     if (c->p.arith == PTD_ARITH_FAST) return fused_ta<T, DC, FORM, kFast>(c, check);
@@ -529,6 +786,7 @@ static hipError_t fused_t(ptd_ctx* c, bool check) {
 hipError_t launch_flux(ptd_ctx* c, int q2mode) { PTD_DISPATCH(flux_t, c, q2mode); }
 hipError_t launch_update(ptd_ctx* c) { PTD_DISPATCH(update_t, c); }
 hipError_t launch_fused(ptd_ctx* c, bool check) { PTD_DISPATCH(fused_t, c, check); }
+hipError_t launch_fused2(ptd_ctx* c, bool check) { PTD_DISPATCH(fused2_t, c, check); }
 
 hipError_t launch_residual(ptd_ctx* c, bool write_resh) {
     const bool f32 = c->esz == 4;
@@ -559,6 +817,8 @@ int64_t partial_capacity(const ptd_params& p, int64_t nR) {
         const int64_t ntx = (p.nx - 2 + 61) / 62, nty = (p.ny - 2 + kRows - 3) / (kRows - 2);
         // smallest chunk is one plane
         nb = std::max<int64_t>(nb, ntx * nty * (p.nz - 2));
+        const int64_t ntx2 = (p.nx - 2 + 59) / 60, nty2 = (p.ny - 2 + kRows - 5) / (kRows - 4);
+        nb = std::max<int64_t>(nb, ntx2 * nty2 * (p.nz - 2));
     }
     return std::max<int64_t>(nb, 1);
 }

@@ -308,3 +308,63 @@ def test_fast_solve_matches_oracle(script, pkg, orc):
     res = pkg.run_script(script, *shape, H0=ref["H0"], D=ref["D"], arith=1, **over)
     assert all(abs(int(a) - int(b)) <= 10 for a, b in zip(res.iters, ref["iters"])), (list(res.iters), list(ref["iters"]))
     assert _relmax(res.H, ref["H"]) <= 1e-10
+
+
+# --------------------------------------------------------------------------------------------------
+# two-level (temporally blocked) sweep: pairs of iterations in one pass
+# --------------------------------------------------------------------------------------------------
+
+@pytest.mark.parametrize("niter", [2, 6, 7])
+@pytest.mark.parametrize("shape", [(37, 29, 21), (131, 19, 47), (64, 16, 5), (3, 3, 3)])
+@pytest.mark.parametrize("script", FUSED_SCRIPTS)
+def test_two_level_sweeps_match_oracle(script, shape, niter, pkg, orc, monkeypatch):
+    """EXACT arithmetic: pairs through the two-level sweep (plus one single sweep for odd counts) give
+    the oracle's values bit for bit, with 4-plane z chunks."""
+    print(f"[gpu] t2 {script} {shape} niter={niter}", flush=True)
+    monkeypatch.setenv("PTD_TZ", "4")
+    monkeypatch.setenv("PTD_T2", "1")
+    params, case, H, Hold, D, q = _random_state(pkg, orc, script, shape, seed=sum(shape) + niter)
+    with pkg.Context(params) as ctx:
+        ctx.set("H", H)
+        ctx.set("Hold", Hold)
+        for k, v in q.items():
+            ctx.set(k, v)
+        dtau = None
+        if D is not None:
+            ctx.set("D", D)
+            dtau = np.zeros(pkg.field_shape(params, 10), order="F")
+            orc.lib().ptdo_iter_params(orc.ctypes.byref(case), orc._p(D), orc._p(dtau))
+        ctx.iterate(niter)
+        Hg = ctx.get("H")
+        qg = {k: ctx.get(k) for k in q}
+    Hr = H.copy(order="F")
+    qr = {k: v.copy(order="F") for k, v in q.items()}
+    orc.lib().ptdo_iterate(orc.ctypes.byref(case), orc._p(Hr), orc._p(Hold), orc._p(D), orc._p(dtau),
+                           orc._p(qr["qHx"]), orc._p(qr["qHy"]), orc._p(qr["qHz"]), niter)
+    assert np.all(np.isfinite(Hr))
+    assert np.array_equal(Hg, Hr), f"H: rel {_relmax(Hg, Hr):.2e}"
+    for k in q:
+        assert np.array_equal(qg[k], qr[k]), f"{k}: rel {_relmax(qg[k], qr[k]):.2e}"
+
+
+@pytest.mark.parametrize("script", FUSED_SCRIPTS)
+def test_two_level_equals_single_sweeps_fast(script, pkg, orc, monkeypatch):
+    """FAST arithmetic: 8 iterations as 4 two-level passes equal 8 single sweeps bit for bit."""
+    shape = (70, 33, 18)
+    out = []
+    for t2 in ("1", "0"):
+        monkeypatch.setenv("PTD_T2", t2)
+        params, case, H, Hold, D, q = _random_state(pkg, orc, script, shape, seed=7)
+        params.arith = 1
+        with pkg.Context(params) as ctx:
+            ctx.set("H", H)
+            ctx.set("Hold", Hold)
+            for k, v in q.items():
+                ctx.set(k, v)
+            if D is not None:
+                ctx.set("D", D)
+            ctx.iterate(8)
+            out.append((ctx.get("H"), [ctx.get(k) for k in q]))
+    assert np.array_equal(out[0][0], out[1][0])
+    for a, b in zip(out[0][1], out[1][1]):
+        assert np.array_equal(a, b)
