@@ -160,9 +160,28 @@ def main():
             iters, errs = ctx.solve(nt)
             sync()
             tts = time.perf_counter() - t0
+            H64 = ctx.get("H")
         out["time_to_solution"] = {"fp64_s": round(tts, 4), "fp64_iters": [int(v) for v in iters],
                                    "tol": sp["tol"], "nt": nt, "final_err": float(errs[-1])}
         log(f"time-to-solution {tts:.3f} s, iterations {list(iters)}")
+        # the same solve with float32 correction sweeps inside float64 defect correction
+        p3 = pkg.make_params(sp, n, n, n, precision=pkg.MIXED, arith=arith)
+        with pkg.Context(p3) as ctx:
+            ctx.set("H", H0)
+            ctx.set("Hold", H0)
+            sync()
+            log("time-to-solution FP32 sweeps + FP64 defect correction")
+            t0 = time.perf_counter()
+            miters, merrs = ctx.solve(nt)
+            sync()
+            tts_m = time.perf_counter() - t0
+            Hm = ctx.get("H")
+        import numpy as np
+        err_m = float(np.max(np.abs(Hm - H64)) / np.max(np.abs(H64)))
+        out["time_to_solution"].update({"mixed_s": round(tts_m, 4), "mixed_iters": [int(v) for v in miters],
+                                        "mixed_final_err": float(merrs[-1]), "speedup": round(tts / tts_m, 3),
+                                        "mixed_vs_fp64_rel_maxnorm": err_m})
+        log(f"mixed {tts_m:.3f} s ({tts / tts_m:.2f}x), sweeps {list(miters)}, max-norm vs FP64 {err_m:.2e}")
 
     if not args.no_cpu:
         log(f"CPU baseline (~{args.cpu_seconds:.0f} s)")

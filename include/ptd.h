@@ -55,7 +55,10 @@ typedef enum ptd_form { PTD_PLAIN = 0, PTD_PERF = 1 } ptd_form;
  *  PTD_FP64 : as the reference (@init_parallel_stencil(..., Float64, ...)).
  *  PTD_FP32 : fields and sweeps in float32.
  *  PTD_MIXED: float32 sweeps inside a float64 defect-correction loop: the residual of the float64
- *             backward-Euler system and the accumulated solution stay float64 on the device. */
+ *             backward-Euler system and the accumulated solution stay float64 on the device (3D
+ *             only). ptd_pt_solve then stops on the float64 residual with the flux taken from the
+ *             current iterate (the consistent residual), and its iteration count is the number of
+ *             float32 correction sweeps. */
 typedef enum ptd_precision { PTD_FP64 = 0, PTD_FP32 = 1, PTD_MIXED = 2 } ptd_precision;
 
 /* Arithmetic of the sweeps.
@@ -125,6 +128,9 @@ int32_t ptd_ctx_destroy(ptd_ctx* ctx);
 int32_t ptd_ctx_set_params(ptd_ctx* ctx, const ptd_params* params);
 /* Device bytes held by the context. */
 int64_t ptd_ctx_device_bytes(const ptd_ctx* ctx);
+/* PTD_MIXED: float64 defect-correction (outer) iterations of the last PT solve (0 otherwise). No
+ * reference counterpart: the reference has no mixed-precision solver. */
+int64_t ptd_ctx_outer_iterations(const ptd_ctx* ctx);
 /* The context's HIP stream (hipStream_t), for callers that time or order work on it. */
 void*   ptd_ctx_stream(ptd_ctx* ctx);
 

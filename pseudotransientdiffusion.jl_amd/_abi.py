@@ -27,6 +27,7 @@ STATUS_NAMES = {OK: "PTD_OK", ERR_ARG: "PTD_ERR_ARG", ERR_HIP: "PTD_ERR_HIP", ER
 EXPORTS = (
     "ptd_abi_version", "ptd_last_error", "ptd_device_check",
     "ptd_ctx_create", "ptd_ctx_destroy", "ptd_ctx_set_params", "ptd_ctx_device_bytes", "ptd_ctx_stream",
+    "ptd_ctx_outer_iterations",
     "ptd_set_field", "ptd_get_field", "ptd_field_count",
     "ptd_compute_iter_params", "ptd_compute_flux", "ptd_compute_update", "ptd_compute_flux_res",
     "ptd_check_res", "ptd_assign",
@@ -79,6 +80,7 @@ def load():
         "ptd_ctx_destroy": (i32, [vp]),
         "ptd_ctx_set_params": (i32, [vp, pp]),
         "ptd_ctx_device_bytes": (i64, [vp]),
+        "ptd_ctx_outer_iterations": (i64, [vp]),
         "ptd_ctx_stream": (vp, [vp]),
         "ptd_set_field": (i32, [vp, i32, dp, i64]),
         "ptd_get_field": (i32, [vp, i32, dp, i64]),

@@ -76,6 +76,11 @@ class Context:
         return int(self._lib.ptd_ctx_device_bytes(self.handle))
 
     @property
+    def outer_iterations(self):
+        """Float64 defect-correction iterations of the last PT solve (PTD_MIXED)."""
+        return int(self._lib.ptd_ctx_outer_iterations(self.handle))
+
+    @property
     def stream(self):
         return self._lib.ptd_ctx_stream(self.handle)
 

@@ -2,6 +2,7 @@
 // kernel-level entry points (one reference kernel each) and the device-resident PT loop.
 #include <hip/hip_runtime.h>
 
+#include <algorithm>
 #include <cmath>
 #include <cstdio>
 #include <cstdlib>
@@ -40,8 +41,10 @@ int check_params(const ptd_params* p) {
     PTD_REQUIRE(p->ndim >= 1 && p->ndim <= 3, PTD_ERR_ARG, "ndim must be 1, 2 or 3");
     PTD_REQUIRE(p->dcase >= PTD_LIN && p->dcase <= PTD_NONLIN, PTD_ERR_ARG, "dcase must be PTD_LIN, PTD_STEP or PTD_NONLIN");
     PTD_REQUIRE(p->form == PTD_PLAIN || p->form == PTD_PERF, PTD_ERR_ARG, "form must be PTD_PLAIN or PTD_PERF");
-    PTD_REQUIRE(p->precision == PTD_FP64 || p->precision == PTD_FP32, PTD_ERR_ARG,
-                "precision must be PTD_FP64 or PTD_FP32 for a context");
+    PTD_REQUIRE(p->precision == PTD_FP64 || p->precision == PTD_FP32 || p->precision == PTD_MIXED, PTD_ERR_ARG,
+                "precision must be PTD_FP64, PTD_FP32 or PTD_MIXED");
+    PTD_REQUIRE(p->precision != PTD_MIXED || p->ndim == 3, PTD_ERR_ARG,
+                "PTD_MIXED runs the fused 3D correction sweeps: ndim must be 3");
     PTD_REQUIRE(p->stop_norm == PTD_NORM_RMS || p->stop_norm == PTD_NORM_MAX, PTD_ERR_ARG, "stop_norm must be PTD_NORM_RMS or PTD_NORM_MAX");
     PTD_REQUIRE(p->arith == PTD_ARITH_EXACT || p->arith == PTD_ARITH_FAST, PTD_ERR_ARG, "arith must be PTD_ARITH_EXACT or PTD_ARITH_FAST");
     PTD_REQUIRE(p->nx >= 3, PTD_ERR_ARG, "nx must be >= 3");
@@ -112,6 +115,13 @@ void free_ctx(ptd_ctx* c) {
         for (int a = 0; a < 3; ++a) (void)hipFree(c->q[a][b]);
     }
     for (int a = 0; a < 3; ++a) (void)hipFree(c->q2[a]);
+    for (int b = 0; b < 2; ++b) {
+        (void)hipFree(c->dl[b]);
+        for (int a = 0; a < 3; ++a) (void)hipFree(c->pq[a][b]);
+    }
+    (void)hipFree(c->src);
+    (void)hipFree(c->Df);
+    (void)hipFree(c->hcf);
     (void)hipFree(c->Hold);
     (void)hipFree(c->D);
     (void)hipFree(c->dtau);
@@ -223,6 +233,85 @@ int run_iterations(ptd_ctx* c, int64_t n, bool check_last) {
     return PTD_OK;
 }
 
+// float32 correction iterations (pairs through the two-level sweep), the last one optionally checked
+int run_delta(ptd_ctx* c, int64_t n, bool check_last) {
+    while (n > 0) {
+        const bool two = n >= 2;
+        const bool chk = check_last && n == (two ? 2 : 1);
+        PTD_HIP(ptd::launch_delta_sweep(c, two, chk));
+        n -= two ? 2 : 1;
+    }
+    return PTD_OK;
+}
+
+// Defect-correction PT solve (PTD_MIXED). Outer loop in float64: R = residual of the backward-Euler
+// system at the iterate Hc (flux from Hc itself); inner loop in float32: PT sweeps on the correction
+// delta with source R (same Re / theta / dtau rules, evaluated at Hc + delta), checked every nout
+// iterations on the float32 correction residual; then Hc += delta. The inner sweeps are the reference
+// PT iteration written in shifted variables (delta = H - Hc, p = q + g(Hc)); they start from the true
+// fluxes q carried over from the previous solve, as the reference's q arrays are (starting from the
+// equilibrium flux of a non-equilibrium H instead overshoots). The inner solve stops once it has
+// reduced its residual kInner-fold (well above the float32 floor) or reached 0.5*tol, or stalls.
+int pt_solve_mixed(ptd_ctx* c, int64_t* iters, double* err_rms, double* err_max) {
+    constexpr double kInner = 1e-5;
+    const ptd_params& p = c->p;
+    if (p.dcase == PTD_STEP) PTD_HIP(ptd::launch_f64_to_f32(c, (float*)c->Df, (const double*)c->D, c->nH));
+    auto residual64 = [&](double* rms, double* mx) -> int {
+        PTD_HIP(ptd::launch_flux(c, 2));
+        PTD_HIP(ptd::launch_residual(c, true));
+        return fetch_errors(c, rms, mx);
+    };
+    double rms = 0, mx = 0;
+    int st = residual64(&rms, &mx);
+    if (st != PTD_OK) return st;
+    double err = p.stop_norm == PTD_NORM_MAX ? mx : rms;
+    int64_t total = 0, outer = 0;
+    while (err > p.tol && total < p.itmax && !std::isnan(err)) {
+        PTD_HIP(ptd::launch_make_source(c));
+        if (p.dcase == PTD_NONLIN)
+            PTD_HIP(ptd::launch_f64_to_f32(c, (float*)c->hcf, (const double*)c->H[c->hcur], c->nH));
+        PTD_HIP(hipMemsetAsync(c->dl[c->dcur], 0, (size_t)c->nH * sizeof(float), c->stream));
+        // start from the true fluxes (as the reference keeps qHx.. across steps): p = q + g(Hc)
+        PTD_HIP(ptd::launch_flux_state(c, true));
+        const double target = std::max(0.5 * p.tol, kInner * err);
+        double best = HUGE_VAL;
+        int stall = 0;
+        int64_t it = 0;
+        while (total + it < p.itmax) {
+            const int64_t n = std::min<int64_t>(p.nout, p.itmax - total - it);
+            st = run_delta(c, n, true);
+            if (st != PTD_OK) return st;
+            it += n;
+            double irms = 0, imx = 0;
+            st = fetch_errors(c, &irms, &imx);
+            if (st != PTD_OK) return st;
+            const double r = p.stop_norm == PTD_NORM_MAX ? imx : irms;
+            if (std::isnan(r) || r <= target) break;
+            if (r < 0.95 * best) {
+                best = r;
+                stall = 0;
+            } else if (++stall >= 3) {
+                break;  // at the float32 floor of this correction
+            }
+        }
+        total += it;
+        PTD_HIP(ptd::launch_flux_state(c, false));  // q = p - g(Hc), before Hc moves
+        PTD_HIP(ptd::launch_add_correction(c));
+        st = residual64(&rms, &mx);
+        if (st != PTD_OK) return st;
+        err = p.stop_norm == PTD_NORM_MAX ? mx : rms;
+        ++outer;
+        if (it == 0) break;
+    }
+    PTD_HIP(hipStreamSynchronize(c->stream));
+    c->outer_last = outer;
+    if (iters) *iters = total;
+    if (err_rms) *err_rms = rms;
+    if (err_max) *err_max = mx;
+    if (std::isnan(err)) return fail(PTD_ERR_NAN, "NaN");
+    return PTD_OK;
+}
+
 }  // namespace
 
 extern "C" {
@@ -261,6 +350,7 @@ int32_t ptd_ctx_create(const ptd_params* params, int32_t device, ptd_ctx** out) 
     if (c->p.ndim < 3) c->p.nz = 1;
     c->device = device;
     c->esz = params->precision == PTD_FP32 ? 4 : 8;
+    c->mixed = params->precision == PTD_MIXED;
     shapes(c->p, c->nH, c->nQ, c->nR);
     hipError_t e = hipStreamCreate(&c->stream);
     if (e != hipSuccess) {
@@ -286,6 +376,15 @@ int32_t ptd_ctx_create(const ptd_params* params, int32_t device, ptd_ctx** out) 
         PTD_ALLOC(&c->dtau, c->nR, c->esz);
     }
     PTD_ALLOC(&c->resh, c->nR, c->esz);
+    if (c->mixed) {
+        for (int b = 0; b < 2; ++b) {
+            PTD_ALLOC(&c->dl[b], c->nH, sizeof(float));
+            for (int a = 0; a < 3; ++a) PTD_ALLOC(&c->pq[a][b], c->nQ[a], sizeof(float));
+        }
+        PTD_ALLOC(&c->src, c->nH, sizeof(float));
+        if (c->p.dcase == PTD_STEP) PTD_ALLOC(&c->Df, c->nH, sizeof(float));
+        if (c->p.dcase == PTD_NONLIN) PTD_ALLOC(&c->hcf, c->nH, sizeof(float));
+    }
     c->npartial = ptd::partial_capacity(c->p, c->nR);
     PTD_ALLOC((void**)&c->partial, 2 * c->npartial, sizeof(double));
     PTD_ALLOC((void**)&c->dred, 2, sizeof(double));
@@ -325,6 +424,8 @@ int32_t ptd_ctx_set_params(ptd_ctx* ctx, const ptd_params* params) {
 }
 
 int64_t ptd_ctx_device_bytes(const ptd_ctx* ctx) { return ctx ? ctx->bytes : 0; }
+
+int64_t ptd_ctx_outer_iterations(const ptd_ctx* ctx) { return ctx ? ctx->outer_last : -1; }
 
 void* ptd_ctx_stream(ptd_ctx* ctx) { return ctx ? (void*)ctx->stream : nullptr; }
 
@@ -446,6 +547,7 @@ int32_t ptd_iterate_timed(ptd_ctx* ctx, int64_t niter, double* ms) {
 int32_t ptd_pt_solve(ptd_ctx* ctx, int64_t* iters, double* err_rms, double* err_max) {
     int st = device_guard(ctx);
     if (st != PTD_OK) return st;
+    if (ctx->mixed) return pt_solve_mixed(ctx, iters, err_rms, err_max);
     const ptd_params& p = ctx->p;
     int64_t iter = 0;
     double err = 2.0 * p.tol, rms = err, mx = err;

@@ -194,10 +194,8 @@ __device__ __forceinline__ double rcp_nr(double x) {
     e = fma(-x, r, 1.0);
     return fma(r, e, r);
 }
-__device__ __forceinline__ float rcp_nr(float x) {
-    float r = __builtin_amdgcn_rcpf(x);
-    return fmaf(r, fmaf(-x, r, 1.0f), r);
-}
+// float32: v_rcp_f32 is within 1 ulp, enough for the float32 sweeps (no Newton step).
+__device__ __forceinline__ float rcp_nr(float x) { return __builtin_amdgcn_rcpf(x); }
 
 // sqrt(x) for normal x > 0: x*rsq(x) with one coupled Newton step on (g, h) = (sqrt x, 1/(2 sqrt x))
 // and a final residual correction.
@@ -215,20 +213,58 @@ __device__ __forceinline__ float sqrt_nr(float x) { return __builtin_amdgcn_sqrt
 template <typename T>
 __device__ __forceinline__ T fmaT(T a, T b, T c) { return fma(a, b, c); }
 
+// H^3 floored at the smallest normal number of T: with the reference's initial condition H reaches
+// 1e-31 near the corners and H^3 underflows to 0 in float32 (0 * inf in the Re terms). The floor only
+// acts where H^3 is not representable; there D ~ 0 either way.
+__device__ __forceinline__ float floor_min(float x) { return fmaxf(x, 1.17549435e-38f); }
+__device__ __forceinline__ double floor_min(double x) { return fmax(x, 2.2250738585072014e-308); }
+// Re = pi + sqrt(pi^2 + K/H^3) capped in float32, where K/H^3 overflows for H^3 near the floor: with
+// Re = 1e30, theta = c1/Re ~ 0 and the face flux goes to its D ~ 0 limit instead of 0*inf.
+__device__ __forceinline__ float cap_re(float x) { return fminf(x, 1e30f); }
+__device__ __forceinline__ double cap_re(double x) { return x; }
+
 // ---------------------------------------------------------------------------------------------
 // one face / one cell, EXACT (reference order) or FAST (rearranged)
 // ---------------------------------------------------------------------------------------------
 
 enum { kExact = PTD_ARITH_EXACT, kFast = PTD_ARITH_FAST };
 
+// Internal D case of the mixed-precision correction sweep for D = H^3: the sweep iterates on the
+// correction delta (float32) around the float64 iterate Hc, whose float32 rounding travels through the
+// kernels in the per-cell "aux" channel (the channel the step case uses for D). The coefficients only
+// need float32 accuracy: the fixed point of the defect correction is set by the float64 residual.
+constexpr int kDelta = 3;
+
+template <typename T, int DC>
+struct AuxOf {
+    using type = T;
+};
+template <int DC>
+constexpr bool has_aux() { return DC == PTD_STEP || DC == kDelta; }
+
 template <typename T, int DC, int FORM, int ARITH>
 struct Phys {
-    // New damped flux on the face (a | b) from the old flux q; g receives the residual flux
-    // (-coef*dH/h, the q2 of the reference) when wanted.
+    using Ta = typename AuxOf<T, DC>::type;
+
+    // New damped flux on the face (a | b) from the old flux q; q2 receives the residual flux
+    // (-coef*dH/h, the q2 of the reference) when wanted. Da, Db: aux values (D, or Hc for kDelta).
     template <bool Q2>
-    __device__ static __forceinline__ T face(const KP<T>& p, T q, T Ha, T Hb, T Da, T Db, T h, T ih, T& q2) {
+    __device__ static __forceinline__ T face(const KP<T>& p, T q, T Ha, T Hb, Ta Da, Ta Db, T h, T ih, T& q2) {
         const T dH = Hb - Ha;
-        if constexpr (ARITH == kExact) {
+        if constexpr (DC == kDelta) {
+            // correction flux around Hc: g(Hc+delta) - g(Hc) = dD*dHc + (hbar+dbar)^3*ddelta with
+            // dD = (hbar+dbar)^3 - hbar^3 = dbar*(3*hbar*(hbar+dbar) + dbar^2), no cancellation
+            const T hbar = (T)0.5 * (Da + Db);
+            const T dHc = Db - Da;
+            const T dbar = (T)0.5 * (Ha + Hb);
+            const T hsum = hbar + dbar;
+            const T dn = floor_min(hsum * hsum * hsum);
+            const T dD = dbar * fmaT((T)3 * hbar, hsum, dbar * dbar);
+            const T g = fmaT(dD, dHc, dn * dH) * ih;
+            const T re = cap_re(p.pi + sqrt_nr(fmaT(p.K, rcp_nr(dn), p.pi2)));
+            if constexpr (Q2) q2 = -g;
+            return fmaT(q, p.c1, -(g * re)) * rcp_nr(re + p.c1);
+        } else if constexpr (ARITH == kExact) {
             const T coef = face_coef<T, DC, FORM>(p, Ha, Hb, Da, Db);
             if constexpr (Q2) q2 = res_flux<T, FORM>(coef, dH, h, ih);
             return damped_flux<T, FORM>(q, face_theta<T, DC, FORM>(p, coef), coef, dH, h, ih);
@@ -236,8 +272,8 @@ struct Phys {
             if constexpr (DC == PTD_NONLIN) {
                 // q' = (q*theta - g)/(1+theta), theta = c1/Re  ==>  q' = (q*c1 - g*Re)/(Re + c1)
                 const T hf = (T)0.5 * (Ha + Hb);
-                const T h3 = hf * hf * hf;
-                const T re = p.pi + sqrt_nr(fmaT(p.K, rcp_nr(h3), p.pi2));
+                const T h3 = floor_min(hf * hf * hf);
+                const T re = cap_re(p.pi + sqrt_nr(fmaT(p.K, rcp_nr(h3), p.pi2)));
                 const T g = h3 * (ih * dH);
                 if constexpr (Q2) q2 = -g;
                 return fmaT(q, p.c1, -(g * re)) * rcp_nr(re + p.c1);
@@ -273,7 +309,7 @@ struct Phys {
 
     template <int NDIM>
     __device__ static __forceinline__ T div(const KP<T>& p, T ddx, T ddy, T ddz) {
-        if constexpr (ARITH == kExact) {
+        if constexpr (ARITH == kExact && DC != kDelta) {
             return divergence<T, FORM, NDIM>(p, ddx, ddy, ddz);
         } else {
             T s = p._dx * ddx;
@@ -283,9 +319,18 @@ struct Phys {
         }
     }
 
-    // H update of one cell from the flux divergence; dmax = @maxloc(D) (step case only).
-    __device__ static __forceinline__ T cell(const KP<T>& p, T H, T Hold, T div, T dmax) {
-        if constexpr (ARITH == kExact) {
+    // H update of one cell from the flux divergence; dmax = @maxloc(D) (step case only); Ac = Hc
+    // (kDelta only).
+    __device__ static __forceinline__ T cell(const KP<T>& p, T H, T Hold, T div, T dmax, Ta Ac = 0) {
+        if constexpr (DC == kDelta) {
+            // correction update with dtau from H = Hc + delta; Hold carries R*dt (source R)
+            const T src = fmaT(p._dt, Hold, -div);
+            const T Hf = (T)Ac + H;
+            const T h3 = floor_min(Hf * Hf * Hf);
+            const T re = cap_re(p.pi + sqrt_nr(fmaT(p.K, rcp_nr(h3), p.pi2)));
+            const T P = h3 * re;
+            return fmaT(H, P, p.c2 * src) * rcp_nr(fmaT(p.c2, p._dt, P));
+        } else if constexpr (ARITH == kExact) {
             T dtr;
             if constexpr (DC == PTD_LIN) dtr = p.dtau_rho;
             else if constexpr (DC == PTD_STEP) dtr = step_dtau(p, dmax);
@@ -301,8 +346,8 @@ struct Phys {
                 return fmaT(H, dmax, p.c3 * src) * rcp_nr(fmaT(p.c3, p._dt, dmax));
             } else {
                 // dtau = c2/(H^3 Re) = c2/P  ==>  (H*P + c2*src)/(P + _dt*c2)
-                const T h3 = H * H * H;
-                const T re = p.pi + sqrt_nr(fmaT(p.K, rcp_nr(h3), p.pi2));
+                const T h3 = floor_min(H * H * H);
+                const T re = cap_re(p.pi + sqrt_nr(fmaT(p.K, rcp_nr(h3), p.pi2)));
                 const T P = h3 * re;
                 return fmaT(H, P, p.c2 * src) * rcp_nr(fmaT(p.c2, p._dt, P));
             }
@@ -310,7 +355,7 @@ struct Phys {
     }
 
     __device__ static __forceinline__ T resid(const KP<T>& p, T H, T Hold, T div) {
-        if constexpr (ARITH == kExact) return residual<T, FORM>(p, H, Hold, div);
+        if constexpr (ARITH == kExact && DC != kDelta) return residual<T, FORM>(p, H, Hold, div);
         else return fmaT(-p._dt, H - Hold, -div);
     }
 };
@@ -341,6 +386,15 @@ struct ptd_ctx {
     double* hred;               // pinned host mirror of dred
     hipEvent_t ev0, ev1;
     int64_t bytes;
+    // PTD_MIXED: float32 correction sweeps around the float64 iterate H[hcur]
+    bool mixed;
+    void* dl[2];                // correction delta (float, nH)
+    void* pq[3][2];             // correction fluxes (float)
+    void* src;                  // R*dt (float, nH): the Hold slot of the correction sweep
+    void* Df;                   // float copy of D (step case)
+    void* hcf;                  // float copy of the float64 iterate (aux of the D = H^3 correction sweep)
+    int dcur, pcur;
+    int64_t outer_last;         // outer (defect-correction) iterations of the last PT solve
 };
 
 namespace ptd {
@@ -353,5 +407,13 @@ hipError_t launch_residual(ptd_ctx* c, bool write_resh);  // partial sums -> dre
 hipError_t launch_fused(ptd_ctx* c, bool check);          // one fused PT iteration (3D)
 hipError_t launch_fused2(ptd_ctx* c, bool check);         // two fused PT iterations in one pass (3D)
 bool fused_available(const ptd_ctx* c);
+// mixed precision: float32 correction sweeps (1 or 2 iterations), source and correction kernels
+hipError_t launch_delta_sweep(ptd_ctx* c, bool two, bool check);
+hipError_t launch_make_source(ptd_ctx* c);      // src = float(ResH * dt) on inner cells
+hipError_t launch_add_correction(ptd_ctx* c);   // H[hcur] += double(delta)
+hipError_t launch_f64_to_f32(ptd_ctx* c, float* dst, const double* src, int64_t n);
+// correction fluxes <-> true fluxes: to_p: p = float(q + g(H)); !to_p: q = double(p) - g(H), where g(H)
+// is the equilibrium flux magnitude coef*dH/h of the float64 iterate H on each face
+hipError_t launch_flux_state(ptd_ctx* c, bool to_p);
 
 }  // namespace ptd

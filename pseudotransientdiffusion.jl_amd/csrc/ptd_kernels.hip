@@ -21,6 +21,9 @@ static constexpr int kBlock = 256;
 __device__ __forceinline__ double nanmax(double a, double b) {
     return (a != a || a > b) ? a : b;
 }
+__device__ __forceinline__ float nanmax(float a, float b) {
+    return (a != a || a > b) ? a : b;
+}
 
 template <typename T>
 __device__ __forceinline__ T load_or0(const T* __restrict__ a, int64_t idx, bool ok) {
@@ -228,10 +231,10 @@ __device__ __forceinline__ void stp(T* __restrict__ plane, uint32_t off, T v) {
 }
 
 // Operands of one plane p of the march (loaded two planes ahead).
-template <typename T, bool STEP>
+template <typename T, typename Ta>
 struct PlaneOps {
     T Hn;      // H(p+1)
-    T Dn;      // D(p+1) (step case)
+    Ta Dn;     // aux(p+1): D (step) or Hc (correction sweep)
     T Ho;      // Hold(p)
     T qx, qy;  // old fluxes on this thread's lower x / y faces at plane p
     T qz;      // old flux on the upper z face (p | p+1)
@@ -240,14 +243,16 @@ struct PlaneOps {
 template <typename T, int DC, int FORM, int ARITH, int R, bool CHECK>
 __global__ void __launch_bounds__(64 * R)
 k_fused3d(const KP<T> p, const T* __restrict__ Hin, T* __restrict__ Hout, const T* __restrict__ Hold,
-          const T* __restrict__ D, const T* __restrict__ qxi, const T* __restrict__ qyi,
+          const typename AuxOf<T, DC>::type* __restrict__ D, const T* __restrict__ qxi, const T* __restrict__ qyi,
           const T* __restrict__ qzi, T* __restrict__ qxo, T* __restrict__ qyo, T* __restrict__ qzo,
           double2* __restrict__ partial, int tz, int ntx, int nty, int ntiles, int per_xcd) {
     using Ph = Phys<T, DC, FORM, ARITH>;
+    using Ta = typename AuxOf<T, DC>::type;
     constexpr bool STEP = DC == PTD_STEP;
+    constexpr bool AUX = has_aux<DC>();
     __shared__ T sH[R][64];
     __shared__ T sQ[R][64];
-    __shared__ T sD[STEP ? R : 1][64];
+    __shared__ Ta sD[AUX ? R : 1][64];
     __shared__ T sQ2[CHECK ? R : 1][64];
 
     // XCD-aware tile order: consecutive blocks go round-robin to the 8 XCDs, so block b takes tile
@@ -290,15 +295,16 @@ k_fused3d(const KP<T> p, const T* __restrict__ Hin, T* __restrict__ Hout, const 
     const int jzq = max(1, min(j, ny - 2));
     constexpr uint32_t es = (uint32_t)sizeof(T);
     const uint32_t oH = (uint32_t)(ic + nx * jc) * es;
+    const uint32_t oA = (uint32_t)(ic + nx * jc) * (uint32_t)sizeof(Ta);
     const uint32_t oQx = (uint32_t)((ixq - 1) + (nx - 1) * (jxq - 1)) * es;
     const uint32_t oQy = (uint32_t)((iyq - 1) + (nx - 2) * (jyq - 1)) * es;
     const uint32_t oQz = (uint32_t)((iyq - 1) + (nx - 2) * (jzq - 1)) * es;
 
     auto load_plane = [&](int pl) {  // operands of plane pl (clamped to the chunk's last plane)
         pl = pl < k1 ? pl : k1 - 1;
-        PlaneOps<T, STEP> o;
+        PlaneOps<T, Ta> o;
         o.Hn = ldp(Hin + psH * (pl + 1), oH, true);
-        if constexpr (STEP) o.Dn = ldp(D + psH * (pl + 1), oH, true);
+        if constexpr (AUX) o.Dn = ldp(D + psH * (pl + 1), oA, true);
         else o.Dn = 0;
         o.Ho = ldp(Hold + psH * pl, oH, true);
         o.qx = ldp(qxi + psQx * (pl - 1), oQx, true);
@@ -310,28 +316,28 @@ k_fused3d(const KP<T> p, const T* __restrict__ Hin, T* __restrict__ Hout, const 
     // lower z face of the first plane
     T Hc = ldp(Hin + psH * k0, oH, true);
     const T Hkm = ldp(Hin + psH * (k0 - 1), oH, true);
-    T Dc = 0, Dkm = 0;
-    if constexpr (STEP) {
-        Dc = ldp(D + psH * k0, oH, true);
-        Dkm = ldp(D + psH * (k0 - 1), oH, true);
+    Ta Dc = 0, Dkm = 0;
+    if constexpr (AUX) {
+        Dc = ldp(D + psH * k0, oA, true);
+        Dkm = ldp(D + psH * (k0 - 1), oA, true);
     }
     T q2zL = 0;
     T qzL = Ph::template face<CHECK>(p, ldp(qzi + psQz * (k0 - 1), oQz, true), Hkm, Hc, Dkm, Dc, p.dz, p._dz, q2zL);
     if (upd && k0 == 1) stp(qzo, oQz, qzL);
 
-    PlaneOps<T, STEP> A = load_plane(k0);
-    PlaneOps<T, STEP> B = load_plane(k0 + 1);
+    PlaneOps<T, Ta> A = load_plane(k0);
+    PlaneOps<T, Ta> B = load_plane(k0 + 1);
 
     double acc = 0.0, amax = 0.0;
     for (int k = k0; k < k1; ++k) {
         sH[row][lane] = Hc;
-        if constexpr (STEP) sD[row][lane] = Dc;
+        if constexpr (AUX) sD[row][lane] = Dc;
         __syncthreads();
 
         const T Hw = __shfl_up(Hc, 1);
         const T Hs = sH[row > 0 ? row - 1 : 0][lane];
-        T Dw = 0, De = 0, Ds = 0, Dnn = 0;
-        if constexpr (STEP) {
+        Ta Dw = 0, De = 0, Ds = 0, Dnn = 0;
+        if constexpr (AUX) {
             Dw = __shfl_up(Dc, 1);
             De = __shfl_down(Dc, 1);
             Ds = sD[row > 0 ? row - 1 : 0][lane];
@@ -354,7 +360,7 @@ k_fused3d(const KP<T> p, const T* __restrict__ Hin, T* __restrict__ Hout, const 
             dmax = fmax(dmax, fmax(Dkm, A.Dn));
         }
         const T dv = Ph::template div<3>(p, qxE - qxn, qyN - qyn, qzn - qzL);
-        const T Hnew = Ph::cell(p, Hc, A.Ho, dv, dmax);
+        const T Hnew = Ph::cell(p, Hc, A.Ho, dv, dmax, Dc);
         if constexpr (CHECK) {
             const T q2xE = __shfl_down(q2x, 1);
             const T q2yN = sQ2[row < R - 1 ? row + 1 : R - 1][lane];
@@ -374,7 +380,7 @@ k_fused3d(const KP<T> p, const T* __restrict__ Hin, T* __restrict__ Hout, const 
         qzL = qzn;
         q2zL = q2z;
         Hc = A.Hn;
-        if constexpr (STEP) {
+        if constexpr (AUX) {
             Dkm = Dc;
             Dc = A.Dn;
         }
@@ -399,29 +405,32 @@ k_fused3d(const KP<T> p, const T* __restrict__ Hin, T* __restrict__ Hout, const 
 // Each level evaluates exactly the expressions of the one-iteration sweep, so two levels here give the
 // same bits as two k_fused3d launches.
 
-template <typename T, bool STEP>
+template <typename T, typename Ta>
 struct PlaneOps2 {
     T Hn;      // H(k+1)
-    T Dn;      // D(k+1) (step)
+    Ta Dn;     // aux(k+1): D (step) or Hc (correction sweep)
     T Ho;      // Hold(k)
     T qx, qy;  // old fluxes on the lower x / y faces at plane k
     T qz;      // old flux on the upper z face (k | k+1)
 };
 
+// float32 sweeps are held to 8 waves per SIMD (two 1024-thread blocks per CU, <= 64 VGPRs)
 template <typename T, int DC, int FORM, int ARITH, int R, bool CHECK>
-__global__ void __launch_bounds__(64 * R)
+__global__ void __launch_bounds__(64 * R, sizeof(T) == 4 ? 8 : 1)
 k_fused3d_t2(const KP<T> p, const T* __restrict__ Hin, T* __restrict__ Hout, const T* __restrict__ Hold,
-             const T* __restrict__ D, const T* __restrict__ qxi, const T* __restrict__ qyi,
+             const typename AuxOf<T, DC>::type* __restrict__ D, const T* __restrict__ qxi, const T* __restrict__ qyi,
              const T* __restrict__ qzi, T* __restrict__ qxo, T* __restrict__ qyo, T* __restrict__ qzo,
              double2* __restrict__ partial, int tz, int ntx, int nty, int ntiles, int per_xcd) {
     using Ph = Phys<T, DC, FORM, ARITH>;
+    using Ta = typename AuxOf<T, DC>::type;
     constexpr bool STEP = DC == PTD_STEP;
+    constexpr bool AUX = has_aux<DC>();
     __shared__ T sH[R][64];                      // H at plane k (level-1 y neighbours)
     __shared__ T sH1[R][64];                     // H1 at plane k-1 (level-2 y neighbours)
     __shared__ T sQ1[R][64];                     // level-1 y flux at plane k
     __shared__ T sQ2[R][64];                     // level-2 y flux at plane k-1
-    __shared__ T sD[STEP ? R : 1][64];           // D at plane k
-    __shared__ T sDp[STEP ? R : 1][64];          // D at plane k-1
+    __shared__ Ta sD[AUX ? R : 1][64];           // aux at plane k
+    __shared__ Ta sDp[AUX ? R : 1][64];          // aux at plane k-1
     __shared__ T sR2[CHECK ? R : 1][64];         // level-2 residual y flux
 
     const int b = blockIdx.x;
@@ -463,17 +472,18 @@ k_fused3d_t2(const KP<T> p, const T* __restrict__ Hin, T* __restrict__ Hout, con
     const int jzq = max(1, min(j, ny - 2));
     constexpr uint32_t es = (uint32_t)sizeof(T);
     const uint32_t oH = (uint32_t)(ic + nx * jc) * es;
+    const uint32_t oA = (uint32_t)(ic + nx * jc) * (uint32_t)sizeof(Ta);
     const uint32_t oQx = (uint32_t)((ixq - 1) + (nx - 1) * (jxq - 1)) * es;
     const uint32_t oQy = (uint32_t)((iyq - 1) + (nx - 2) * (jyq - 1)) * es;
     const uint32_t oQz = (uint32_t)((iyq - 1) + (nx - 2) * (jzq - 1)) * es;
 
     auto load_plane = [&](int pl) {  // level-1 operands of plane pl, pl in [0, nz-1]
-        PlaneOps2<T, STEP> o;
+        PlaneOps2<T, Ta> o;
         const int ph = pl + 1 < nz ? pl + 1 : nz - 1;
         const int pq = pl - 1 < 0 ? 0 : (pl - 1 > nz - 3 ? nz - 3 : pl - 1);
         const int pz = pl > nz - 2 ? nz - 2 : pl;
         o.Hn = ldp(Hin + psH * ph, oH, true);
-        if constexpr (STEP) o.Dn = ldp(D + psH * ph, oH, true);
+        if constexpr (AUX) o.Dn = ldp(D + psH * ph, oA, true);
         else o.Dn = 0;
         o.Ho = ldp(Hold + psH * pl, oH, true);
         o.qx = ldp(qxi + psQx * pq, oQx, true);
@@ -485,29 +495,29 @@ k_fused3d_t2(const KP<T> p, const T* __restrict__ Hin, T* __restrict__ Hout, con
     // level-1 lower z face of plane ka (face (ka-1 | ka)); needed only when plane ka is inner
     T Hc = ldp(Hin + psH * ka, oH, true);
     const int kb = ka > 0 ? ka - 1 : 0;
-    T Dc = 0, Dm1 = 0, Dm2 = 0;  // D at planes k, k-1, k-2
-    if constexpr (STEP) {
-        Dc = ldp(D + psH * ka, oH, true);
-        Dm1 = ldp(D + psH * kb, oH, true);
+    Ta Dc = 0, Dm1 = 0, Dm2 = 0;  // aux at planes k, k-1, k-2
+    if constexpr (AUX) {
+        Dc = ldp(D + psH * ka, oA, true);
+        Dm1 = ldp(D + psH * kb, oA, true);
     }
     T unused = 0;
     T q1zL = Ph::template face<false>(p, ldp(qzi + psQz * kb, oQz, true), ldp(Hin + psH * kb, oH, true), Hc,
                                       Dm1, Dc, p.dz, p._dz, unused);
 
-    PlaneOps2<T, STEP> A = load_plane(ka);
-    PlaneOps2<T, STEP> B = load_plane(ka + 1 <= nz - 1 ? ka + 1 : nz - 1);
+    PlaneOps2<T, Ta> A = load_plane(ka);
+    PlaneOps2<T, Ta> B = load_plane(ka + 1 <= nz - 1 ? ka + 1 : nz - 1);
 
     T H1p = 0;             // H1 at plane k-1
     T Hop = 0;             // Hold at plane k-1
     T q1xp = 0, q1yp = 0;  // level-1 fluxes on the lower x / y faces at plane k-1
     T q2zL = 0, r2zL = 0;  // level-2 flux (and residual flux) on the lower z face of plane k-1
-    double acc = 0.0, amax = 0.0;
+    T acc = 0, amax = 0;   // per-thread sum of squares / max of the residual (in the sweep precision)
 
     for (int k = ka; k <= k1; ++k) {
         // phase A: planes into LDS
         sH[row][lane] = Hc;
         sH1[row][lane] = H1p;
-        if constexpr (STEP) {
+        if constexpr (AUX) {
             sD[row][lane] = Dc;
             sDp[row][lane] = Dm1;
         }
@@ -517,8 +527,8 @@ k_fused3d_t2(const KP<T> p, const T* __restrict__ Hin, T* __restrict__ Hout, con
         const int rs = row > 0 ? row - 1 : 0, rn = row < R - 1 ? row + 1 : R - 1;
         const T Hw = __shfl_up(Hc, 1);
         const T Hs = sH[rs][lane];
-        T Dw = 0, De = 0, Ds = 0, Dnn = 0, Dpw = 0, Dpe = 0, Dps = 0, Dpn = 0;
-        if constexpr (STEP) {
+        Ta Dw = 0, De = 0, Ds = 0, Dnn = 0, Dpw = 0, Dpe = 0, Dps = 0, Dpn = 0;
+        if constexpr (AUX) {
             Dw = __shfl_up(Dc, 1);
             De = __shfl_down(Dc, 1);
             Ds = sD[rs][lane];
@@ -554,7 +564,7 @@ k_fused3d_t2(const KP<T> p, const T* __restrict__ Hin, T* __restrict__ Hout, con
             dmax2 = fmax(dmax2, fmax(Dm2, Dc));
         }
         const T dv1 = Ph::template div<3>(p, q1xE - q1x, q1yN - q1y, q1zU - q1zL);
-        const T H1c = Ph::cell(p, Hc, A.Ho, dv1, dmax1);
+        const T H1c = Ph::cell(p, Hc, A.Ho, dv1, dmax1, Dc);
         const T H1 = (u1 && k >= 1 && k <= nz - 2) ? H1c : Hc;  // boundary cells keep H
 
         T r2z = 0;
@@ -564,12 +574,12 @@ k_fused3d_t2(const KP<T> p, const T* __restrict__ Hin, T* __restrict__ Hout, con
             const T q2xE = __shfl_down(q2x, 1);
             const T q2yN = sQ2[rn][lane];
             const T dv2 = Ph::template div<3>(p, q2xE - q2x, q2yN - q2y, q2zU - q2zL);
-            const T H2 = Ph::cell(p, H1p, Hop, dv2, dmax2);
+            const T H2 = Ph::cell(p, H1p, Hop, dv2, dmax2, Dm1);
             if constexpr (CHECK) {
                 const T r2xE = __shfl_down(r2x, 1);
                 const T r2yN = sR2[rn][lane];
                 const T dvr = Ph::template div<3>(p, r2xE - r2x, r2yN - r2y, r2z - r2zL);
-                const double rr = u2 ? (double)Ph::resid(p, H2, Hop, dvr) : 0.0;
+                const T rr = u2 ? Ph::resid(p, H2, Hop, dvr) : (T)0;
                 acc += rr * rr;
                 amax = nanmax(fabs(rr), amax);
             }
@@ -590,7 +600,7 @@ k_fused3d_t2(const KP<T> p, const T* __restrict__ Hin, T* __restrict__ Hout, con
         Hop = A.Ho;
         q1xp = q1x;
         q1yp = q1y;
-        if constexpr (STEP) {
+        if constexpr (AUX) {
             Dm2 = Dm1;
             Dm1 = Dc;
             Dc = A.Dn;
@@ -599,7 +609,7 @@ k_fused3d_t2(const KP<T> p, const T* __restrict__ Hin, T* __restrict__ Hout, con
         A = B;
         B = load_plane(k + 2 <= nz - 1 ? k + 2 : nz - 1);
     }
-    if constexpr (CHECK) block_sum_max(acc, amax, row * 64 + lane, 64 * R, &partial[tile]);
+    if constexpr (CHECK) block_sum_max((double)acc, (double)amax, row * 64 + lane, 64 * R, &partial[tile]);
 }
 
 // ---------------------------------------------------------------------------------------------
@@ -684,83 +694,94 @@ static int fused_tz(const ptd_ctx* c) {
     return (int)std::min<int64_t>(16, mz);
 }
 
+// Buffers of one sweep: iterate n in, iterate n+1 (or n+2) out.
+struct SweepBufs {
+    const void* Hin;
+    void* Hout;
+    const void* Hold;
+    const void* aux;       // D (step), Hc (correction sweep of D = H^3), unused otherwise
+    const void* qin[3];
+    void* qout[3];
+};
+
+// One fused sweep (two = false: one iteration, k_fused3d) or a two-level pass (two iterations,
+// k_fused3d_t2); with `check` the residual of the last iteration is reduced into c->dred.
 template <typename T, int DC, int FORM, int ARITH>
-static hipError_t fused_ta(ptd_ctx* c, bool check) {
+static hipError_t sweep_run(ptd_ctx* c, const SweepBufs& b, bool two, bool check) {
+    using Ta = typename AuxOf<T, DC>::type;
     const KP<T> kp = make_kp<T>(c->p);
     const int tz = fused_tz(c);
-    const int ntx = (int)((c->p.nx - 2 + 61) / 62);
-    const int nty = (int)((c->p.ny - 2 + kRows - 3) / (kRows - 2));
+    const int w = two ? 60 : 62, rows = two ? kRows - 4 : kRows - 2;
+    const int ntx = (int)((c->p.nx - 2 + w - 1) / w);
+    const int nty = (int)((c->p.ny - 2 + rows - 1) / rows);
     const int nbz = (int)((c->p.nz - 2 + tz - 1) / tz);
     const int ntiles = ntx * nty * nbz;
     const int per = (ntiles + 7) / 8;
     const dim3 g(8 * per), bl(64, kRows);
-    const int h = c->hcur, q = c->qcur;
-    const T* Hin = (const T*)c->H[h];
-    T* Hout = (T*)c->H[1 - h];
-    const T* Ho = (const T*)c->Hold;
-    const T* D = (const T*)c->D;
-    if (check)
-        hipLaunchKernelGGL((k_fused3d<T, DC, FORM, ARITH, kRows, true>), g, bl, 0, c->stream, kp, Hin, Hout, Ho, D,
-                           (const T*)c->q[0][q], (const T*)c->q[1][q], (const T*)c->q[2][q], (T*)c->q[0][1 - q],
-                           (T*)c->q[1][1 - q], (T*)c->q[2][1 - q], (double2*)c->partial, tz, ntx, nty, ntiles, per);
-    else
-        hipLaunchKernelGGL((k_fused3d<T, DC, FORM, ARITH, kRows, false>), g, bl, 0, c->stream, kp, Hin, Hout, Ho, D,
-                           (const T*)c->q[0][q], (const T*)c->q[1][q], (const T*)c->q[2][q], (T*)c->q[0][1 - q],
-                           (T*)c->q[1][1 - q], (T*)c->q[2][1 - q], (double2*)c->partial, tz, ntx, nty, ntiles, per);
+    const T* Hin = (const T*)b.Hin;
+    T* Hout = (T*)b.Hout;
+    const T* Ho = (const T*)b.Hold;
+    const Ta* A = (const Ta*)b.aux;
+    const T* q0 = (const T*)b.qin[0];
+    const T* q1 = (const T*)b.qin[1];
+    const T* q2 = (const T*)b.qin[2];
+    T* o0 = (T*)b.qout[0];
+    T* o1 = (T*)b.qout[1];
+    T* o2 = (T*)b.qout[2];
+    double2* part = (double2*)c->partial;
+    if (two) {
+        if (check)
+            hipLaunchKernelGGL((k_fused3d_t2<T, DC, FORM, ARITH, kRows, true>), g, bl, 0, c->stream, kp, Hin, Hout,
+                               Ho, A, q0, q1, q2, o0, o1, o2, part, tz, ntx, nty, ntiles, per);
+        else
+            hipLaunchKernelGGL((k_fused3d_t2<T, DC, FORM, ARITH, kRows, false>), g, bl, 0, c->stream, kp, Hin, Hout,
+                               Ho, A, q0, q1, q2, o0, o1, o2, part, tz, ntx, nty, ntiles, per);
+    } else {
+        if (check)
+            hipLaunchKernelGGL((k_fused3d<T, DC, FORM, ARITH, kRows, true>), g, bl, 0, c->stream, kp, Hin, Hout, Ho,
+                               A, q0, q1, q2, o0, o1, o2, part, tz, ntx, nty, ntiles, per);
+        else
+            hipLaunchKernelGGL((k_fused3d<T, DC, FORM, ARITH, kRows, false>), g, bl, 0, c->stream, kp, Hin, Hout, Ho,
+                               A, q0, q1, q2, o0, o1, o2, part, tz, ntx, nty, ntiles, per);
+    }
     hipError_t e = hipGetLastError();
     if (e != hipSuccess) return e;
-    c->hcur = 1 - h;
-    c->qcur = 1 - q;
     if (check)
         hipLaunchKernelGGL(k_reduce_partials, dim3(1), dim3(kBlock), 0, c->stream, (const double2*)c->partial,
                            (int64_t)ntiles, c->dred);
     return hipGetLastError();
 }
 
-template <typename T, int DC, int FORM, int ARITH>
-static hipError_t fused2_ta(ptd_ctx* c, bool check) {
-    const KP<T> kp = make_kp<T>(c->p);
-    const int tz = fused_tz(c);
-    const int ntx = (int)((c->p.nx - 2 + 59) / 60);
-    const int nty = (int)((c->p.ny - 2 + kRows - 5) / (kRows - 4));
-    const int nbz = (int)((c->p.nz - 2 + tz - 1) / tz);
-    const int ntiles = ntx * nty * nbz;
-    const int per = (ntiles + 7) / 8;
-    const dim3 g(8 * per), bl(64, kRows);
+// The context's own fields (H, q double-buffered; the sweep flips them).
+static SweepBufs ctx_bufs(ptd_ctx* c) {
     const int h = c->hcur, q = c->qcur;
-    const T* Hin = (const T*)c->H[h];
-    T* Hout = (T*)c->H[1 - h];
-    const T* Ho = (const T*)c->Hold;
-    const T* D = (const T*)c->D;
-    if (check)
-        hipLaunchKernelGGL((k_fused3d_t2<T, DC, FORM, ARITH, kRows, true>), g, bl, 0, c->stream, kp, Hin, Hout, Ho, D,
-                           (const T*)c->q[0][q], (const T*)c->q[1][q], (const T*)c->q[2][q], (T*)c->q[0][1 - q],
-                           (T*)c->q[1][1 - q], (T*)c->q[2][1 - q], (double2*)c->partial, tz, ntx, nty, ntiles, per);
-    else
-        hipLaunchKernelGGL((k_fused3d_t2<T, DC, FORM, ARITH, kRows, false>), g, bl, 0, c->stream, kp, Hin, Hout, Ho, D,
-                           (const T*)c->q[0][q], (const T*)c->q[1][q], (const T*)c->q[2][q], (T*)c->q[0][1 - q],
-                           (T*)c->q[1][1 - q], (T*)c->q[2][1 - q], (double2*)c->partial, tz, ntx, nty, ntiles, per);
-    hipError_t e = hipGetLastError();
+    SweepBufs b;
+    b.Hin = c->H[h];
+    b.Hout = c->H[1 - h];
+    b.Hold = c->Hold;
+    b.aux = c->D;
+    for (int a = 0; a < 3; ++a) {
+        b.qin[a] = c->q[a][q];
+        b.qout[a] = c->q[a][1 - q];
+    }
+    return b;
+}
+
+template <typename T, int DC, int FORM, bool TWO>
+static hipError_t ctx_sweep_t(ptd_ctx* c, bool check) {
+    const SweepBufs b = ctx_bufs(c);
+    hipError_t e = c->p.arith == PTD_ARITH_FAST ? sweep_run<T, DC, FORM, kFast>(c, b, TWO, check)
+                                                : sweep_run<T, DC, FORM, kExact>(c, b, TWO, check);
     if (e != hipSuccess) return e;
-    c->hcur = 1 - h;
-    c->qcur = 1 - q;
-    if (check)
-        hipLaunchKernelGGL(k_reduce_partials, dim3(1), dim3(kBlock), 0, c->stream, (const double2*)c->partial,
-                           (int64_t)ntiles, c->dred);
-    return hipGetLastError();
+    c->hcur = 1 - c->hcur;
+    c->qcur = 1 - c->qcur;
+    return hipSuccess;
 }
 
 template <typename T, int DC, int FORM>
-static hipError_t fused2_t(ptd_ctx* c, bool check) {
-    if (c->p.arith == PTD_ARITH_FAST) return fused2_ta<T, DC, FORM, kFast>(c, check);
-    return fused2_ta<T, DC, FORM, kExact>(c, check);
-}
-
+static hipError_t fused_t(ptd_ctx* c, bool check) { return ctx_sweep_t<T, DC, FORM, false>(c, check); }
 template <typename T, int DC, int FORM>
-static hipError_t fused_t(ptd_ctx* c, bool check) {
-    if (c->p.arith == PTD_ARITH_FAST) return fused_ta<T, DC, FORM, kFast>(c, check);
-    return fused_ta<T, DC, FORM, kExact>(c, check);
-}
+static hipError_t fused2_t(ptd_ctx* c, bool check) { return ctx_sweep_t<T, DC, FORM, true>(c, check); }
 
 // dispatch on (precision, dcase, form)
 #define PTD_DISPATCH(FN, c, ...)                                                                        \
@@ -808,6 +829,148 @@ hipError_t launch_iter_params(ptd_ctx* c) {
 
 bool fused_available(const ptd_ctx* c) {
     return c->p.ndim == 3 && c->p.nx >= 3 && c->p.ny >= 3 && c->p.nz >= 3;
+}
+
+// ---------------------------------------------------------------------------------------------
+// mixed precision: float32 correction sweeps around a float64 iterate
+// ---------------------------------------------------------------------------------------------
+
+// src = float(R * dt) on the inner cells (R = ResH of the float64 iterate).
+__global__ void __launch_bounds__(kBlock)
+k_make_source(const KP<double> p, const double* __restrict__ resh, float* __restrict__ src, double dt, int64_t count) {
+    const int64_t t = (int64_t)blockIdx.x * kBlock + threadIdx.x;
+    if (t >= count) return;
+    const CellIdx c = cell_index(p, t);
+    src[c.h] = (float)(resh[t] * dt);
+}
+
+// H += double(delta) on the inner cells.
+__global__ void __launch_bounds__(kBlock)
+k_add_correction(const KP<double> p, double* __restrict__ H, const float* __restrict__ dl, int64_t count) {
+    const int64_t t = (int64_t)blockIdx.x * kBlock + threadIdx.x;
+    if (t >= count) return;
+    const CellIdx c = cell_index(p, t);
+    H[c.h] += (double)dl[c.h];
+}
+
+__global__ void __launch_bounds__(kBlock)
+k_f64_to_f32(float* __restrict__ dst, const double* __restrict__ src, int64_t n) {
+    const int64_t t = (int64_t)blockIdx.x * kBlock + threadIdx.x;
+    if (t < n) dst[t] = (float)src[t];
+}
+
+hipError_t launch_make_source(ptd_ctx* c) {
+    const KP<double> kp = make_kp<double>(c->p);
+    hipLaunchKernelGGL(k_make_source, dim3(grid_for(c->nR)), dim3(kBlock), 0, c->stream, kp,
+                       (const double*)c->resh, (float*)c->src, c->p.dt, c->nR);
+    return hipGetLastError();
+}
+
+hipError_t launch_add_correction(ptd_ctx* c) {
+    const KP<double> kp = make_kp<double>(c->p);
+    hipLaunchKernelGGL(k_add_correction, dim3(grid_for(c->nR)), dim3(kBlock), 0, c->stream, kp,
+                       (double*)c->H[c->hcur], (const float*)c->dl[c->dcur], c->nR);
+    return hipGetLastError();
+}
+
+hipError_t launch_f64_to_f32(ptd_ctx* c, float* dst, const double* src, int64_t n) {
+    hipLaunchKernelGGL(k_f64_to_f32, dim3(grid_for(n)), dim3(kBlock), 0, c->stream, dst, src, n);
+    return hipGetLastError();
+}
+
+// Face state conversion between the true fluxes q (float64, they persist over physical steps as the
+// reference's qHx/qHy/qHz do) and the correction fluxes p = q + g(Hc) (float32) of the correction
+// sweep, g(Hc) = coef * dHc / h the equilibrium flux magnitude of the float64 iterate.
+template <int DC, int AXIS, bool TO_P>
+__global__ void __launch_bounds__(kBlock)
+k_flux_state(const KP<double> p, const double* __restrict__ H, const double* __restrict__ D,
+             double* __restrict__ q, float* __restrict__ pf, int64_t count) {
+    const int64_t t = (int64_t)blockIdx.x * kBlock + threadIdx.x;
+    if (t >= count) return;
+    int64_t a, step;
+    double ih;
+    if constexpr (AXIS == 0) {
+        const int64_t nf = p.nx - 1, f = t % nf, r = t / nf, jj = r % p.my, kk = r / p.my;
+        a = f + p.nx * ((jj + p.hy) + p.ny * (kk + p.hz));
+        step = 1;
+        ih = p._dx;
+    } else if constexpr (AXIS == 1) {
+        const int64_t nf = p.ny - 1, ii = t % p.mx, r = t / p.mx, f = r % nf, kk = r / nf;
+        a = (ii + 1) + p.nx * (f + p.ny * (kk + p.hz));
+        step = p.nx;
+        ih = p._dy;
+    } else {
+        const int64_t ii = t % p.mx, r = t / p.mx, jj = r % p.my, f = r / p.my;
+        a = (ii + 1) + p.nx * ((jj + 1) + p.ny * f);
+        step = p.nx * p.ny;
+        ih = p._dz;
+    }
+    const double Ha = H[a], Hb = H[a + step];
+    double coef;
+    if constexpr (DC == PTD_NONLIN) {
+        const double hf = 0.5 * (Ha + Hb);
+        coef = hf * hf * hf;
+    } else if constexpr (DC == PTD_STEP) {
+        coef = (D[a] + D[a + step]) * 0.5;
+    } else {
+        coef = p.d0;
+    }
+    const double g = coef * (ih * (Hb - Ha));
+    if constexpr (TO_P) pf[t] = (float)(q[t] + g);
+    else q[t] = (double)pf[t] - g;
+}
+
+template <int DC, bool TO_P>
+static hipError_t flux_state_t(ptd_ctx* c) {
+    const KP<double> kp = make_kp<double>(c->p);
+    const double* H = (const double*)c->H[c->hcur];
+    const double* D = (const double*)c->D;
+    for (int a = 0; a < 3; ++a) {
+        const int64_t n = c->nQ[a];
+        if (n == 0) continue;
+        double* q = (double*)c->q[a][c->qcur];
+        float* pf = (float*)c->pq[a][c->pcur];
+        const dim3 g(grid_for(n)), bl(kBlock);
+        if (a == 0) hipLaunchKernelGGL((k_flux_state<DC, 0, TO_P>), g, bl, 0, c->stream, kp, H, D, q, pf, n);
+        else if (a == 1) hipLaunchKernelGGL((k_flux_state<DC, 1, TO_P>), g, bl, 0, c->stream, kp, H, D, q, pf, n);
+        else hipLaunchKernelGGL((k_flux_state<DC, 2, TO_P>), g, bl, 0, c->stream, kp, H, D, q, pf, n);
+    }
+    return hipGetLastError();
+}
+
+hipError_t launch_flux_state(ptd_ctx* c, bool to_p) {
+    if (c->p.dcase == PTD_NONLIN) return to_p ? flux_state_t<PTD_NONLIN, true>(c) : flux_state_t<PTD_NONLIN, false>(c);
+    if (c->p.dcase == PTD_STEP) return to_p ? flux_state_t<PTD_STEP, true>(c) : flux_state_t<PTD_STEP, false>(c);
+    return to_p ? flux_state_t<PTD_LIN, true>(c) : flux_state_t<PTD_LIN, false>(c);
+}
+
+// One or two float32 correction iterations on (delta, correction fluxes) with source src = R*dt in
+// the Hold slot; for D = H^3 float32(Hc) rides in the aux channel.
+hipError_t launch_delta_sweep(ptd_ctx* c, bool two, bool check) {
+    SweepBufs b;
+    const int d = c->dcur, q = c->pcur;
+    b.Hin = c->dl[d];
+    b.Hout = c->dl[1 - d];
+    b.Hold = c->src;
+    for (int a = 0; a < 3; ++a) {
+        b.qin[a] = c->pq[a][q];
+        b.qout[a] = c->pq[a][1 - q];
+    }
+    hipError_t e;
+    if (c->p.dcase == PTD_NONLIN) {
+        b.aux = c->hcf;
+        e = sweep_run<float, kDelta, PTD_PERF, kFast>(c, b, two, check);
+    } else if (c->p.dcase == PTD_STEP) {
+        b.aux = c->Df;
+        e = sweep_run<float, PTD_STEP, PTD_PERF, kFast>(c, b, two, check);
+    } else {
+        b.aux = nullptr;
+        e = sweep_run<float, PTD_LIN, PTD_PERF, kFast>(c, b, two, check);
+    }
+    if (e != hipSuccess) return e;
+    c->dcur = 1 - d;
+    c->pcur = 1 - q;
+    return hipSuccess;
 }
 
 // Number of (sum,max) partials the reductions can produce for this context.
