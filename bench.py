@@ -29,6 +29,39 @@ def log(msg):
     print(f"[bench] {msg}", file=sys.stderr, flush=True)
 
 
+TRAFFIC_PROFILE = os.path.join(ROOT, "profiles", "fused3d_t2_nonlin_511_traffic.md")
+
+
+def calib_copy():
+    """Counter calibration: three plain float64 copies of exactly 1 GiB (read 1 GiB, write 1 GiB each),
+    so that FETCH_SIZE / WRITE_SIZE of a profiled run can be scaled to bytes."""
+    import torch
+    n = 1 << 27
+    a = torch.ones(n, dtype=torch.float64, device="cuda")
+    b = torch.empty_like(a)
+    for _ in range(3):
+        b.copy_(a)
+    torch.cuda.synchronize()
+    print(json.dumps({"calib_copy_bytes_read": n * 8, "calib_copy_bytes_written": n * 8, "launches": 3}))
+    return 0
+
+
+def measured_traffic(n, dtype, arith):
+    """HBM bytes per launch of the two-level sweep, from the committed rocprofv3 FETCH_SIZE / WRITE_SIZE
+    summary (line `traffic: n=<n> dtype=<dtype> arith=<arith> read=<B> write=<B>`), or None."""
+    try:
+        lines = open(TRAFFIC_PROFILE).read().splitlines()
+    except OSError:
+        return None
+    for ln in lines:
+        if not ln.startswith("traffic:"):
+            continue
+        kv = dict(t.split("=", 1) for t in ln.split()[1:])
+        if kv.get("n") == str(n) and kv.get("dtype") == dtype and kv.get("arith") == arith:
+            return int(kv["read"]) + int(kv["write"])
+    return None
+
+
 def cpu_baseline(pkg, n, seconds):
     """The oracle's PT iterations (OpenMP C restatement of the reference Threads path) at n^3 on this
     host: run iterations until `seconds` have elapsed (at least one), report T_eff."""
@@ -73,7 +106,11 @@ def main():
     ap.add_argument("--no-cpu", action="store_true", help="skip the CPU baseline leg")
     ap.add_argument("--no-tts", action="store_true", help="skip the time-to-solution leg")
     ap.add_argument("--cpu-seconds", type=float, default=15.0)
+    ap.add_argument("--calib-copy", action="store_true",
+                    help="only run the 1 GiB copy used to scale rocprofv3 FETCH_SIZE/WRITE_SIZE, then exit")
     args = ap.parse_args()
+    if args.calib_copy:
+        return calib_copy()
 
     world = int(os.environ.get("WORLD_SIZE", "1"))
     if args.gpus != 1 or world != 1:
@@ -121,6 +158,11 @@ def main():
             ms_exact = None
     t_it = ms / 1e3 / args.steps
     t_eff = a_eff / t_it
+    # the dominant kernel is the two-level sweep: one launch = two PT iterations (an even step count
+    # is all pairs), so its algorithmic bytes per launch are 2*A_eff and its launch time 2*t_it
+    # (achieved = 2*A_eff / (2*t_it) = T_eff)
+    iters_per_launch = 2 if (args.steps % 2 == 0 and os.environ.get("PTD_T2", "1") != "0") else 1
+    traffic = measured_traffic(n, "f64" if prec == pkg.FP64 else "f32", args.arith)
     log(f"t_it = {t_it * 1e3:.4f} ms (events), wall {wall / args.steps * 1e3:.4f} ms, T_eff = {t_eff:.1f} GB/s")
 
     out = {
@@ -138,9 +180,12 @@ def main():
         "data": "synthetic",
         "config": {"workload": f"3D nonlinear D=H^3 PT iteration (fused flux+update sweep), {n}^3, "
                                f"perf-script expressions, H0=1+2exp(-r^2), H=1 boundaries",
-                   "nx": n, "ny": n, "nz": n, "A_eff_GB_per_iter": round(a_eff, 6), "arith": args.arith},
+                   "nx": n, "ny": n, "nz": n, "A_eff_GB_per_iter": round(a_eff, 6), "arith": args.arith,
+                   "kernel": "k_fused3d_t2" if iters_per_launch == 2 else "k_fused3d",
+                   "iterations_per_launch": iters_per_launch,
+                   "launch_us": round(t_it * iters_per_launch * 1e6, 2)},
         "roofline": {"bound": "hbm", "achieved": round(t_eff, 2), "peak": PEAK_HBM_GBS, "unit": "GB/s",
-                     "frac": round(t_eff / PEAK_HBM_GBS, 4), "traffic": None},
+                     "frac": round(t_eff / PEAK_HBM_GBS, 4), "traffic": traffic},
     }
 
     if ms_exact is not None:

@@ -186,29 +186,19 @@ __device__ __forceinline__ T residual(const KP<T>& p, T H, T Hold, T div) {
 // FAST arithmetic: Newton-refined hardware reciprocal / reciprocal square root
 // ---------------------------------------------------------------------------------------------
 
-// 1/x: v_rcp_f64 then two Newton steps (relative error below 2 ulp for normal x).
-__device__ __forceinline__ double rcp_nr(double x) {
-    double r = __builtin_amdgcn_rcp(x);
-    double e = fma(-x, r, 1.0);
-    r = fma(r, e, r);
-    e = fma(-x, r, 1.0);
-    return fma(r, e, r);
+// One Newton step on v_rcp_f64 / v_rsq_f64 (raw relative error ~5e-8, after the step ~4e-15). Used
+// only for the damping coefficients of the FAST update rules below, whose fixed point does not depend
+// on them; float32 sweeps use the hardware value (within 1 ulp of float32).
+__device__ __forceinline__ double rcp_1nr(double x) {
+    const double r = __builtin_amdgcn_rcp(x);
+    return fma(r, fma(-x, r, 1.0), r);
 }
-// float32: v_rcp_f32 is within 1 ulp, enough for the float32 sweeps (no Newton step).
-__device__ __forceinline__ float rcp_nr(float x) { return __builtin_amdgcn_rcpf(x); }
-
-// sqrt(x) for normal x > 0: x*rsq(x) with one coupled Newton step on (g, h) = (sqrt x, 1/(2 sqrt x))
-// and a final residual correction.
-__device__ __forceinline__ double sqrt_nr(double x) {
+__device__ __forceinline__ float rcp_1nr(float x) { return __builtin_amdgcn_rcpf(x); }
+__device__ __forceinline__ double rsq_1nr(double x) {
     const double y = __builtin_amdgcn_rsq(x);
-    double g = x * y, h = 0.5 * y;
-    const double r = fma(-g, h, 0.5);
-    g = fma(g, r, g);
-    h = fma(h, r, h);
-    const double d = fma(-g, g, x);
-    return fma(d, h, g);
+    return fma(0.5 * y, fma(-(x * y), y, 1.0), y);
 }
-__device__ __forceinline__ float sqrt_nr(float x) { return __builtin_amdgcn_sqrtf(x); }
+__device__ __forceinline__ float rsq_1nr(float x) { return __builtin_amdgcn_rsqf(x); }
 
 template <typename T>
 __device__ __forceinline__ T fmaT(T a, T b, T c) { return fma(a, b, c); }
@@ -222,6 +212,15 @@ __device__ __forceinline__ double floor_min(double x) { return fmax(x, 2.2250738
 // Re = 1e30, theta = c1/Re ~ 0 and the face flux goes to its D ~ 0 limit instead of 0*inf.
 __device__ __forceinline__ float cap_re(float x) { return fminf(x, 1e30f); }
 __device__ __forceinline__ double cap_re(double x) { return x; }
+
+// FAST Re = pi + sqrt(pi^2 + K/h3) = pi + a*rsq(a*h3) with a = pi^2*h3 + K: one reciprocal square root
+// instead of a reciprocal and a square root. a*h3 stays finite and normal for h3 in [floor, 1e150]
+// (K = max_l^2/dt > 0).
+template <typename T>
+__device__ __forceinline__ T re_fast(const KP<T>& p, T h3) {
+    const T a = fmaT(p.pi2, h3, p.K);
+    return cap_re(p.pi + a * rsq_1nr(a * h3));
+}
 
 // ---------------------------------------------------------------------------------------------
 // one face / one cell, EXACT (reference order) or FAST (rearranged)
@@ -261,22 +260,23 @@ struct Phys {
             const T dn = floor_min(hsum * hsum * hsum);
             const T dD = dbar * fmaT((T)3 * hbar, hsum, dbar * dbar);
             const T g = fmaT(dD, dHc, dn * dH) * ih;
-            const T re = cap_re(p.pi + sqrt_nr(fmaT(p.K, rcp_nr(dn), p.pi2)));
+            const T re = re_fast(p, dn);
             if constexpr (Q2) q2 = -g;
-            return fmaT(q, p.c1, -(g * re)) * rcp_nr(re + p.c1);
+            return fmaT(-(q + g), re * rcp_1nr(re + p.c1), q);
         } else if constexpr (ARITH == kExact) {
             const T coef = face_coef<T, DC, FORM>(p, Ha, Hb, Da, Db);
             if constexpr (Q2) q2 = res_flux<T, FORM>(coef, dH, h, ih);
             return damped_flux<T, FORM>(q, face_theta<T, DC, FORM>(p, coef), coef, dH, h, ih);
         } else {
             if constexpr (DC == PTD_NONLIN) {
-                // q' = (q*theta - g)/(1+theta), theta = c1/Re  ==>  q' = (q*c1 - g*Re)/(Re + c1)
+                // q' = (q*theta - g)/(1+theta) with theta = c1/Re is q' = q - (q + g)*w, w = Re/(Re + c1):
+                // the fixed point q = -g does not depend on the rounding of w
                 const T hf = (T)0.5 * (Ha + Hb);
                 const T h3 = floor_min(hf * hf * hf);
-                const T re = cap_re(p.pi + sqrt_nr(fmaT(p.K, rcp_nr(h3), p.pi2)));
+                const T re = re_fast(p, h3);
                 const T g = h3 * (ih * dH);
                 if constexpr (Q2) q2 = -g;
-                return fmaT(q, p.c1, -(g * re)) * rcp_nr(re + p.c1);
+                return fmaT(-(q + g), re * rcp_1nr(re + p.c1), q);
             } else {
                 T coef;
                 if constexpr (DC == PTD_LIN) coef = p.d0;
@@ -321,15 +321,14 @@ struct Phys {
 
     // H update of one cell from the flux divergence; dmax = @maxloc(D) (step case only); Ac = Hc
     // (kDelta only).
-    __device__ static __forceinline__ T cell(const KP<T>& p, T H, T Hold, T div, T dmax, Ta Ac = 0) {
+    __device__ static __forceinline__ T cell(const KP<T>& p, T H, T Hold, T div, T dmax, Ta Ac = Ta{}) {
         if constexpr (DC == kDelta) {
             // correction update with dtau from H = Hc + delta; Hold carries R*dt (source R)
             const T src = fmaT(p._dt, Hold, -div);
-            const T Hf = (T)Ac + H;
+            const T Hf = Ac + H;
             const T h3 = floor_min(Hf * Hf * Hf);
-            const T re = cap_re(p.pi + sqrt_nr(fmaT(p.K, rcp_nr(h3), p.pi2)));
-            const T P = h3 * re;
-            return fmaT(H, P, p.c2 * src) * rcp_nr(fmaT(p.c2, p._dt, P));
+            const T P = h3 * re_fast(p, h3);
+            return fmaT(fmaT(-H, p._dt, src), p.c2 * rcp_1nr(fmaT(p.c2, p._dt, P)), H);
         } else if constexpr (ARITH == kExact) {
             T dtr;
             if constexpr (DC == PTD_LIN) dtr = p.dtau_rho;
@@ -338,18 +337,18 @@ struct Phys {
             return h_update<T, FORM>(p, H, Hold, dtr, div);
         } else {
             const T src = fmaT(p._dt, Hold, -div);
+            // (H + dtau*src)/(1 + _dt*dtau) = H + (src - _dt*H)*v, v = dtau/(1 + _dt*dtau): src - _dt*H is
+            // the residual, so the fixed point does not depend on the rounding of v
             if constexpr (DC == PTD_LIN) {
-                // (H + dtau*src)/(1 + _dt*dtau)
                 return fmaT(p.dtau_rho, src, H) * p.inv1p_dtau;
             } else if constexpr (DC == PTD_STEP) {
-                // dtau = c3/m  ==>  (H*m + c3*src)/(m + _dt*c3)
-                return fmaT(H, dmax, p.c3 * src) * rcp_nr(fmaT(p.c3, p._dt, dmax));
+                // dtau = c3/m  ==>  v = c3/(m + _dt*c3)
+                return fmaT(fmaT(-H, p._dt, src), p.c3 * rcp_1nr(fmaT(p.c3, p._dt, dmax)), H);
             } else {
-                // dtau = c2/(H^3 Re) = c2/P  ==>  (H*P + c2*src)/(P + _dt*c2)
+                // dtau = c2/(H^3 Re) = c2/P  ==>  v = c2/(P + _dt*c2)
                 const T h3 = floor_min(H * H * H);
-                const T re = cap_re(p.pi + sqrt_nr(fmaT(p.K, rcp_nr(h3), p.pi2)));
-                const T P = h3 * re;
-                return fmaT(H, P, p.c2 * src) * rcp_nr(fmaT(p.c2, p._dt, P));
+                const T P = h3 * re_fast(p, h3);
+                return fmaT(fmaT(-H, p._dt, src), p.c2 * rcp_1nr(fmaT(p.c2, p._dt, P)), H);
             }
         }
     }

@@ -230,6 +230,12 @@ __device__ __forceinline__ void stp(T* __restrict__ plane, uint32_t off, T v) {
     *reinterpret_cast<T*>(reinterpret_cast<char*>(plane) + off) = v;
 }
 
+// Lane shuffles by one lane (x neighbours).
+template <typename V>
+__device__ __forceinline__ V shfl_up1(V v) { return __shfl_up(v, 1); }
+template <typename V>
+__device__ __forceinline__ V shfl_down1(V v) { return __shfl_down(v, 1); }
+
 // Operands of one plane p of the march (loaded two planes ahead).
 template <typename T, typename Ta>
 struct PlaneOps {
@@ -305,7 +311,7 @@ k_fused3d(const KP<T> p, const T* __restrict__ Hin, T* __restrict__ Hout, const 
         PlaneOps<T, Ta> o;
         o.Hn = ldp(Hin + psH * (pl + 1), oH, true);
         if constexpr (AUX) o.Dn = ldp(D + psH * (pl + 1), oA, true);
-        else o.Dn = 0;
+        else o.Dn = Ta{};
         o.Ho = ldp(Hold + psH * pl, oH, true);
         o.qx = ldp(qxi + psQx * (pl - 1), oQx, true);
         o.qy = ldp(qyi + psQy * (pl - 1), oQy, true);
@@ -316,7 +322,7 @@ k_fused3d(const KP<T> p, const T* __restrict__ Hin, T* __restrict__ Hout, const 
     // lower z face of the first plane
     T Hc = ldp(Hin + psH * k0, oH, true);
     const T Hkm = ldp(Hin + psH * (k0 - 1), oH, true);
-    Ta Dc = 0, Dkm = 0;
+    Ta Dc{}, Dkm{};
     if constexpr (AUX) {
         Dc = ldp(D + psH * k0, oA, true);
         Dkm = ldp(D + psH * (k0 - 1), oA, true);
@@ -336,10 +342,10 @@ k_fused3d(const KP<T> p, const T* __restrict__ Hin, T* __restrict__ Hout, const 
 
         const T Hw = __shfl_up(Hc, 1);
         const T Hs = sH[row > 0 ? row - 1 : 0][lane];
-        Ta Dw = 0, De = 0, Ds = 0, Dnn = 0;
+        Ta Dw{}, De{}, Ds{}, Dnn{};
         if constexpr (AUX) {
-            Dw = __shfl_up(Dc, 1);
-            De = __shfl_down(Dc, 1);
+            Dw = shfl_up1(Dc);
+            De = shfl_down1(Dc);
             Ds = sD[row > 0 ? row - 1 : 0][lane];
             Dnn = sD[row < R - 1 ? row + 1 : R - 1][lane];
         }
@@ -414,9 +420,30 @@ struct PlaneOps2 {
     T qz;      // old flux on the upper z face (k | k+1)
 };
 
-// float32 sweeps are held to 8 waves per SIMD (two 1024-thread blocks per CU, <= 64 VGPRs)
+// Occupancy (measured at 511^3): two blocks per CU without spills is what pays. float32: 16 rows,
+// operands prefetched one plane ahead, <= 64 VGPRs (8 waves per SIMD). float64 FAST lin / nonlin:
+// 12 rows, two planes ahead, <= 80 VGPRs (6 waves per SIMD). The float64 variants with more live
+// values (checking, step D, EXACT) would spill under that cap and run 16 rows, one block per CU.
+template <typename T, int DC, int ARITH, bool CHECK>
+constexpr bool t2_compact() {
+    return sizeof(T) == 8 && ARITH == kFast && DC != PTD_STEP && !CHECK;
+}
+template <typename T, int DC, int ARITH, int R, bool CHECK>
+constexpr int t2_waves_per_simd() {
+    return sizeof(T) == 4 ? 8 : (t2_compact<T, DC, ARITH, CHECK>() ? 2 * (R / 4) : 1);
+}
+template <typename T>
+constexpr int t2_prefetch_depth() {
+    return sizeof(T) == 4 ? 1 : 2;
+}
+template <typename T, int DC, int ARITH, bool CHECK>
+constexpr int t2_rows() {
+    return t2_compact<T, DC, ARITH, CHECK>() ? 12 : 16;
+}
+constexpr int kMinRows2 = 12;  // smallest two-level block (sizes the partials buffer)
+
 template <typename T, int DC, int FORM, int ARITH, int R, bool CHECK>
-__global__ void __launch_bounds__(64 * R, sizeof(T) == 4 ? 8 : 1)
+__global__ void __launch_bounds__(64 * R, (t2_waves_per_simd<T, DC, ARITH, R, CHECK>()))
 k_fused3d_t2(const KP<T> p, const T* __restrict__ Hin, T* __restrict__ Hout, const T* __restrict__ Hold,
              const typename AuxOf<T, DC>::type* __restrict__ D, const T* __restrict__ qxi, const T* __restrict__ qyi,
              const T* __restrict__ qzi, T* __restrict__ qxo, T* __restrict__ qyo, T* __restrict__ qzo,
@@ -443,6 +470,15 @@ k_fused3d_t2(const KP<T> p, const T* __restrict__ Hin, T* __restrict__ Hout, con
 
     const int lane = threadIdx.x, row = threadIdx.y;
     const int nx = (int)p.nx, ny = (int)p.ny, nz = (int)p.nz;
+    // One row per wavefront (blockDim.x == 64): rows whose results no computed cell needs skip the
+    // arithmetic with wave-uniform branches. Level 1: rows 1..R-2 update (faces + cell), row R-1 only
+    // evaluates its lower y face (row R-2's upper one). Level 2: rows 2..R-3 update, row R-2 only
+    // evaluates its lower y face. Row 0 only loads.
+    const int wrow = __builtin_amdgcn_readfirstlane(row);
+    const bool l1all = wrow >= 1 && wrow <= R - 2;
+    const bool l1y = wrow >= 1;
+    const bool l2all = wrow >= 2 && wrow <= R - 3;
+    const bool l2y = wrow >= 2 && wrow <= R - 2;
     const int i = 60 * bx - 1 + lane;
     const int j = (R - 4) * by - 1 + row;
     const int k0 = 1 + tz * bz;
@@ -484,7 +520,7 @@ k_fused3d_t2(const KP<T> p, const T* __restrict__ Hin, T* __restrict__ Hout, con
         const int pz = pl > nz - 2 ? nz - 2 : pl;
         o.Hn = ldp(Hin + psH * ph, oH, true);
         if constexpr (AUX) o.Dn = ldp(D + psH * ph, oA, true);
-        else o.Dn = 0;
+        else o.Dn = Ta{};
         o.Ho = ldp(Hold + psH * pl, oH, true);
         o.qx = ldp(qxi + psQx * pq, oQx, true);
         o.qy = ldp(qyi + psQy * pq, oQy, true);
@@ -495,7 +531,7 @@ k_fused3d_t2(const KP<T> p, const T* __restrict__ Hin, T* __restrict__ Hout, con
     // level-1 lower z face of plane ka (face (ka-1 | ka)); needed only when plane ka is inner
     T Hc = ldp(Hin + psH * ka, oH, true);
     const int kb = ka > 0 ? ka - 1 : 0;
-    Ta Dc = 0, Dm1 = 0, Dm2 = 0;  // aux at planes k, k-1, k-2
+    Ta Dc{}, Dm1{}, Dm2{};  // aux at planes k, k-1, k-2
     if constexpr (AUX) {
         Dc = ldp(D + psH * ka, oA, true);
         Dm1 = ldp(D + psH * kb, oA, true);
@@ -504,8 +540,10 @@ k_fused3d_t2(const KP<T> p, const T* __restrict__ Hin, T* __restrict__ Hout, con
     T q1zL = Ph::template face<false>(p, ldp(qzi + psQz * kb, oQz, true), ldp(Hin + psH * kb, oH, true), Hc,
                                       Dm1, Dc, p.dz, p._dz, unused);
 
+    constexpr bool kAhead2 = t2_prefetch_depth<T>() == 2;
     PlaneOps2<T, Ta> A = load_plane(ka);
-    PlaneOps2<T, Ta> B = load_plane(ka + 1 <= nz - 1 ? ka + 1 : nz - 1);
+    PlaneOps2<T, Ta> B;
+    if constexpr (kAhead2) B = load_plane(ka + 1 <= nz - 1 ? ka + 1 : nz - 1);
 
     T H1p = 0;             // H1 at plane k-1
     T Hop = 0;             // Hold at plane k-1
@@ -527,25 +565,28 @@ k_fused3d_t2(const KP<T> p, const T* __restrict__ Hin, T* __restrict__ Hout, con
         const int rs = row > 0 ? row - 1 : 0, rn = row < R - 1 ? row + 1 : R - 1;
         const T Hw = __shfl_up(Hc, 1);
         const T Hs = sH[rs][lane];
-        Ta Dw = 0, De = 0, Ds = 0, Dnn = 0, Dpw = 0, Dpe = 0, Dps = 0, Dpn = 0;
+        Ta Dw{}, De{}, Ds{}, Dnn{}, Dpw{}, Dpe{}, Dps{}, Dpn{};
         if constexpr (AUX) {
-            Dw = __shfl_up(Dc, 1);
-            De = __shfl_down(Dc, 1);
+            Dw = shfl_up1(Dc);
+            De = shfl_down1(Dc);
             Ds = sD[rs][lane];
             Dnn = sD[rn][lane];
-            Dpw = __shfl_up(Dm1, 1);
-            Dpe = __shfl_down(Dm1, 1);
+            Dpw = shfl_up1(Dm1);
+            Dpe = shfl_down1(Dm1);
             Dps = sDp[rs][lane];
             Dpn = sDp[rn][lane];
         }
-        const T q1x = Ph::template face<false>(p, A.qx, Hw, Hc, Dw, Dc, p.dx, p._dx, unused);
-        const T q1y = Ph::template face<false>(p, A.qy, Hs, Hc, Ds, Dc, p.dy, p._dy, unused);
-        const T q1zU = Ph::template face<false>(p, A.qz, Hc, A.Hn, Dc, A.Dn, p.dz, p._dz, unused);
+        T q1x = 0, q1y = 0, q1zU = 0;
+        if (l1y) q1y = Ph::template face<false>(p, A.qy, Hs, Hc, Ds, Dc, p.dy, p._dy, unused);
+        if (l1all) {
+            q1x = Ph::template face<false>(p, A.qx, Hw, Hc, Dw, Dc, p.dx, p._dx, unused);
+            q1zU = Ph::template face<false>(p, A.qz, Hc, A.Hn, Dc, A.Dn, p.dz, p._dz, unused);
+        }
         const T H1w = __shfl_up(H1p, 1);
         const T H1s = sH1[rs][lane];
-        T r2x = 0, r2y = 0;
-        const T q2x = Ph::template face<CHECK>(p, q1xp, H1w, H1p, Dpw, Dm1, p.dx, p._dx, r2x);
-        const T q2y = Ph::template face<CHECK>(p, q1yp, H1s, H1p, Dps, Dm1, p.dy, p._dy, r2y);
+        T r2x = 0, r2y = 0, q2x = 0, q2y = 0;
+        if (l2y) q2y = Ph::template face<CHECK>(p, q1yp, H1s, H1p, Dps, Dm1, p.dy, p._dy, r2y);
+        if (l2all) q2x = Ph::template face<CHECK>(p, q1xp, H1w, H1p, Dpw, Dm1, p.dx, p._dx, r2x);
         sQ1[row][lane] = q1y;
         sQ2[row][lane] = q2y;
         if constexpr (CHECK) sR2[row][lane] = r2y;
@@ -563,31 +604,36 @@ k_fused3d_t2(const KP<T> p, const T* __restrict__ Hin, T* __restrict__ Hout, con
             dmax2 = fmax(dmax2, fmax(Dps, Dpn));
             dmax2 = fmax(dmax2, fmax(Dm2, Dc));
         }
-        const T dv1 = Ph::template div<3>(p, q1xE - q1x, q1yN - q1y, q1zU - q1zL);
-        const T H1c = Ph::cell(p, Hc, A.Ho, dv1, dmax1, Dc);
-        const T H1 = (u1 && k >= 1 && k <= nz - 2) ? H1c : Hc;  // boundary cells keep H
+        T H1 = Hc;  // boundary cells and rows without a level-1 update keep H
+        if (l1all) {
+            const T dv1 = Ph::template div<3>(p, q1xE - q1x, q1yN - q1y, q1zU - q1zL);
+            const T H1c = Ph::cell(p, Hc, A.Ho, dv1, dmax1, Dc);
+            if (u1 && k >= 1 && k <= nz - 2) H1 = H1c;
+        }
 
-        T r2z = 0;
-        const T q2zU = Ph::template face<CHECK>(p, q1zL, H1p, H1, Dm1, Dc, p.dz, p._dz, r2z);
+        T r2z = 0, q2zU = 0;
+        if (l2all) q2zU = Ph::template face<CHECK>(p, q1zL, H1p, H1, Dm1, Dc, p.dz, p._dz, r2z);
         if (k > k0) {
             const int m = k - 1;
-            const T q2xE = __shfl_down(q2x, 1);
-            const T q2yN = sQ2[rn][lane];
-            const T dv2 = Ph::template div<3>(p, q2xE - q2x, q2yN - q2y, q2zU - q2zL);
-            const T H2 = Ph::cell(p, H1p, Hop, dv2, dmax2, Dm1);
-            if constexpr (CHECK) {
-                const T r2xE = __shfl_down(r2x, 1);
-                const T r2yN = sR2[rn][lane];
-                const T dvr = Ph::template div<3>(p, r2xE - r2x, r2yN - r2y, r2z - r2zL);
-                const T rr = u2 ? Ph::resid(p, H2, Hop, dvr) : (T)0;
-                acc += rr * rr;
-                amax = nanmax(fabs(rr), amax);
+            if (l2all) {
+                const T q2xE = __shfl_down(q2x, 1);
+                const T q2yN = sQ2[rn][lane];
+                const T dv2 = Ph::template div<3>(p, q2xE - q2x, q2yN - q2y, q2zU - q2zL);
+                const T H2 = Ph::cell(p, H1p, Hop, dv2, dmax2, Dm1);
+                if constexpr (CHECK) {
+                    const T r2xE = __shfl_down(r2x, 1);
+                    const T r2yN = sR2[rn][lane];
+                    const T dvr = Ph::template div<3>(p, r2xE - r2x, r2yN - r2y, r2z - r2zL);
+                    const T rr = u2 ? Ph::resid(p, H2, Hop, dvr) : (T)0;
+                    acc += rr * rr;
+                    amax = nanmax(fabs(rr), amax);
+                }
+                if (u2) {
+                    stp(Hout + psH * m, oH, H2);
+                    stp(qzo + psQz * m, oQz, q2zU);
+                }
+                if (w2x) stp(qxo + psQx * (m - 1), oQx, q2x);
             }
-            if (u2) {
-                stp(Hout + psH * m, oH, H2);
-                stp(qzo + psQz * m, oQz, q2zU);
-            }
-            if (w2x) stp(qxo + psQx * (m - 1), oQx, q2x);
             if (w2y) stp(qyo + psQy * (m - 1), oQy, q2y);
         } else if (k == k0 && k0 == 1) {
             if (u2) stp(qzo, oQz, q2zU);  // face (0 | 1): lower face of the first inner plane
@@ -606,8 +652,12 @@ k_fused3d_t2(const KP<T> p, const T* __restrict__ Hin, T* __restrict__ Hout, con
             Dc = A.Dn;
         }
         Hc = A.Hn;
-        A = B;
-        B = load_plane(k + 2 <= nz - 1 ? k + 2 : nz - 1);
+        if constexpr (kAhead2) {
+            A = B;
+            B = load_plane(k + 2 <= nz - 1 ? k + 2 : nz - 1);
+        } else {
+            A = load_plane(k + 1 <= nz - 1 ? k + 1 : nz - 1);
+        }
     }
     if constexpr (CHECK) block_sum_max((double)acc, (double)amax, row * 64 + lane, 64 * R, &partial[tile]);
 }
@@ -682,7 +732,7 @@ static hipError_t residual_t(ptd_ctx* c, bool write_resh) {
     return hipGetLastError();
 }
 
-static constexpr int kRows = 16;  // rows (waves) per fused 3D block
+static constexpr int kRows = 16;  // rows (waves) per block of the one-iteration 3D sweep
 
 // Planes per z chunk. Measured at 511^3 FP64 (FAST): 8 -> 2.20 ms, 16 -> 2.17, 32 -> 2.21, 64 -> 2.30,
 // 128 -> 2.45, 509 -> 3.01 ms per sweep: short chunks give many more blocks than CUs (small tail) and
@@ -711,13 +761,15 @@ static hipError_t sweep_run(ptd_ctx* c, const SweepBufs& b, bool two, bool check
     using Ta = typename AuxOf<T, DC>::type;
     const KP<T> kp = make_kp<T>(c->p);
     const int tz = fused_tz(c);
-    const int w = two ? 60 : 62, rows = two ? kRows - 4 : kRows - 2;
+    constexpr int R2 = t2_rows<T, DC, ARITH, false>(), R2C = t2_rows<T, DC, ARITH, true>();
+    const int brows = two ? (check ? R2C : R2) : kRows;
+    const int w = two ? 60 : 62, rows = two ? brows - 4 : brows - 2;
     const int ntx = (int)((c->p.nx - 2 + w - 1) / w);
     const int nty = (int)((c->p.ny - 2 + rows - 1) / rows);
     const int nbz = (int)((c->p.nz - 2 + tz - 1) / tz);
     const int ntiles = ntx * nty * nbz;
     const int per = (ntiles + 7) / 8;
-    const dim3 g(8 * per), bl(64, kRows);
+    const dim3 g(8 * per), bl(64, brows);
     const T* Hin = (const T*)b.Hin;
     T* Hout = (T*)b.Hout;
     const T* Ho = (const T*)b.Hold;
@@ -731,10 +783,10 @@ static hipError_t sweep_run(ptd_ctx* c, const SweepBufs& b, bool two, bool check
     double2* part = (double2*)c->partial;
     if (two) {
         if (check)
-            hipLaunchKernelGGL((k_fused3d_t2<T, DC, FORM, ARITH, kRows, true>), g, bl, 0, c->stream, kp, Hin, Hout,
+            hipLaunchKernelGGL((k_fused3d_t2<T, DC, FORM, ARITH, R2C, true>), g, bl, 0, c->stream, kp, Hin, Hout,
                                Ho, A, q0, q1, q2, o0, o1, o2, part, tz, ntx, nty, ntiles, per);
         else
-            hipLaunchKernelGGL((k_fused3d_t2<T, DC, FORM, ARITH, kRows, false>), g, bl, 0, c->stream, kp, Hin, Hout,
+            hipLaunchKernelGGL((k_fused3d_t2<T, DC, FORM, ARITH, R2, false>), g, bl, 0, c->stream, kp, Hin, Hout,
                                Ho, A, q0, q1, q2, o0, o1, o2, part, tz, ntx, nty, ntiles, per);
     } else {
         if (check)
@@ -980,7 +1032,7 @@ int64_t partial_capacity(const ptd_params& p, int64_t nR) {
         const int64_t ntx = (p.nx - 2 + 61) / 62, nty = (p.ny - 2 + kRows - 3) / (kRows - 2);
         // smallest chunk is one plane
         nb = std::max<int64_t>(nb, ntx * nty * (p.nz - 2));
-        const int64_t ntx2 = (p.nx - 2 + 59) / 60, nty2 = (p.ny - 2 + kRows - 5) / (kRows - 4);
+        const int64_t ntx2 = (p.nx - 2 + 59) / 60, nty2 = (p.ny - 2 + kMinRows2 - 5) / (kMinRows2 - 4);
         nb = std::max<int64_t>(nb, ntx2 * nty2 * (p.nz - 2));
     }
     return std::max<int64_t>(nb, 1);
