@@ -62,6 +62,18 @@ def measured_traffic(n, dtype, arith):
     return None
 
 
+def solve_steps(ctx, nt, tag):
+    """nt x {PT solve; Hold .= H} -- what ptd_solve does in one call -- with a log line per step."""
+    import numpy as np
+    iters, errs = np.zeros(nt, dtype=np.int64), np.zeros(nt)
+    for it in range(nt):
+        iters[it], rms, mx = ctx.pt_solve()
+        errs[it] = rms
+        ctx.assign()
+        log(f"  {tag} step {it + 1}/{nt}: {iters[it]} iterations, err {rms:.3e}")
+    return iters, errs
+
+
 def cpu_baseline(pkg, n, seconds):
     """The oracle's PT iterations (OpenMP C restatement of the reference Threads path) at n^3 on this
     host: run iterations until `seconds` have elapsed (at least one), report T_eff."""
@@ -106,6 +118,8 @@ def main():
     ap.add_argument("--no-cpu", action="store_true", help="skip the CPU baseline leg")
     ap.add_argument("--no-tts", action="store_true", help="skip the time-to-solution leg")
     ap.add_argument("--cpu-seconds", type=float, default=15.0)
+    ap.add_argument("--tts-n", type=int, default=0,
+                    help="grid size of the time-to-solution legs (default: --n; the precision workload is 1023)")
     ap.add_argument("--calib-copy", action="store_true",
                     help="only run the 1 GiB copy used to scale rocprofv3 FETCH_SIZE/WRITE_SIZE, then exit")
     args = ap.parse_args()
@@ -138,8 +152,12 @@ def main():
     a_eff = pkg.a_eff_gb(3, (n, n, n), esz)
 
     with pkg.Context(params) as ctx:
+        sync()
+        t_up = time.perf_counter()
         ctx.set("H", H0)
         ctx.set("Hold", H0)
+        sync()
+        t_up = time.perf_counter() - t_up
         log(f"warmup {args.warmup} iterations")
         ctx.iterate(args.warmup)
         sync()
@@ -148,6 +166,9 @@ def main():
         ms = ctx.iterate_timed(args.steps)
         sync()
         wall = time.perf_counter() - t0
+        t_down = time.perf_counter()
+        ctx.get("H")
+        t_down = time.perf_counter() - t_down
         if args.arith == "fast":
             # the same sweep in the reference's exact operation order, for comparison
             p_exact = pkg.make_params(spec, n, n, n, precision=prec, arith=0)
@@ -188,13 +209,24 @@ def main():
                      "frac": round(t_eff / PEAK_HBM_GBS, 4), "traffic": traffic},
     }
 
+    # the boundary takes host arrays: the same K iterations including the upload of H, Hold and the
+    # download of H over PCIe (reported beside `value`, which starts with the fields resident in HBM)
+    t_pcie = ms / 1e3 + t_up + t_down
+    out["pcie_inclusive"] = {"T_eff": round(a_eff * args.steps / t_pcie, 2), "upload_s": round(t_up, 4),
+                             "download_s": round(t_down, 4),
+                             "bytes": int(3 * n ** 3 * esz)}
+
     if ms_exact is not None:
         out["exact_arith"] = {"ms_per_step": round(ms_exact, 5), "T_eff": round(a_eff / (ms_exact / 1e3), 2)}
 
     if not args.no_tts:
         # time-to-solution: converging solve, 5 physical steps, tol 1e-8 on the RMS residual, nout 10
         sp = dict(pkg.SCRIPTS["diff_3D_nonlin_multixpu"])
-        p2 = pkg.make_params(sp, n, n, n, precision=pkg.FP64, arith=arith)
+        tn = args.tts_n or n
+        if tn != n:
+            log(f"building config IC at {tn}^3")
+            H0 = pkg.initial_condition(spec, tn, tn, tn, kind="config", amp=2.0, bval=1.0)
+        p2 = pkg.make_params(sp, tn, tn, tn, precision=pkg.FP64, arith=arith)
         nt = int(math.ceil(sp["ttot"] / sp["dt"]))
         with pkg.Context(p2) as ctx:
             ctx.set("H", H0)
@@ -202,22 +234,22 @@ def main():
             sync()
             log("time-to-solution FP64 (5 steps, tol 1e-8)")
             t0 = time.perf_counter()
-            iters, errs = ctx.solve(nt)
+            iters, errs = solve_steps(ctx, nt, "fp64")
             sync()
             tts = time.perf_counter() - t0
             H64 = ctx.get("H")
-        out["time_to_solution"] = {"fp64_s": round(tts, 4), "fp64_iters": [int(v) for v in iters],
+        out["time_to_solution"] = {"n": tn, "fp64_s": round(tts, 4), "fp64_iters": [int(v) for v in iters],
                                    "tol": sp["tol"], "nt": nt, "final_err": float(errs[-1])}
         log(f"time-to-solution {tts:.3f} s, iterations {list(iters)}")
         # the same solve with float32 correction sweeps inside float64 defect correction
-        p3 = pkg.make_params(sp, n, n, n, precision=pkg.MIXED, arith=arith)
+        p3 = pkg.make_params(sp, tn, tn, tn, precision=pkg.MIXED, arith=arith)
         with pkg.Context(p3) as ctx:
             ctx.set("H", H0)
             ctx.set("Hold", H0)
             sync()
             log("time-to-solution FP32 sweeps + FP64 defect correction")
             t0 = time.perf_counter()
-            miters, merrs = ctx.solve(nt)
+            miters, merrs = solve_steps(ctx, nt, "mixed")
             sync()
             tts_m = time.perf_counter() - t0
             Hm = ctx.get("H")

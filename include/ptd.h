@@ -64,10 +64,12 @@ typedef enum ptd_precision { PTD_FP64 = 0, PTD_FP32 = 1, PTD_MIXED = 2 } ptd_pre
 /* Arithmetic of the sweeps.
  *  PTD_ARITH_EXACT: every expression in the reference's operation order with IEEE division and
  *                   square root (bit-identical to the reference kernels' Float64 results).
- *  PTD_ARITH_FAST : the same update rules rearranged algebraically (one reciprocal per face and per
- *                   cell instead of up to four divisions, Newton-refined hardware reciprocal and
- *                   reciprocal square root); agrees with EXACT to a few ulp per iteration and to
- *                   < 1e-10 relative on converged solutions. Requires H > 0 in the nonlinear case. */
+ *  PTD_ARITH_FAST : the same update rules rearranged algebraically, q' = q - (q + g)*Re/(Re + c1) and
+ *                   H' = H + R*dtau/(1 + dtau/dt) with R the residual, so that the damping
+ *                   coefficients (computed from hardware reciprocal / reciprocal square root with one
+ *                   Newton step, ~4e-15 relative) do not move the fixed point; agrees with EXACT to
+ *                   ~1e-14 per iteration and to < 1e-10 relative on converged solutions. Requires
+ *                   H > 0 in the nonlinear case. */
 typedef enum ptd_arith { PTD_ARITH_EXACT = 0, PTD_ARITH_FAST = 1 } ptd_arith;
 
 /* Norm that stops the PT loop. The reference stops on RMS = norm(ResH)/sqrt(length(ResH))

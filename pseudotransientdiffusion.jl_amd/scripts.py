@@ -279,6 +279,39 @@ def _env_bool(key, default):
     return default if v is None else v.strip().lower() in ("1", "true", "yes")
 
 
+def viz_file_name(name):
+    """File name of the script's MAT dump: the multi-xPU 3D scripts drop the `_multixpu` suffix
+    (scripts/diff_3D/diff_3D_nonlin_multixpu.jl:137), the others keep their name
+    (scripts/diff_1D_lin.jl:92, scripts/diff_3D/diff_3D_nonlin_perf.jl:133)."""
+    if name.endswith("_multixpu"):
+        name = name[: -len("_multixpu")]
+    return name + ".mat"
+
+
+def save_viz(name, result, out_dir="out_visu"):
+    """The scripts' DO_SAVE_VIZ dump: a compressed MAT v5 file with H_<n>D and the cell-centre
+    coordinates xc_<n>D, yc_<n>D, zc_<n>D (vectors as n x 1 columns, as MAT.jl writes Julia vectors).
+    The multi-xPU scripts save the gathered inner array inn(H) (H_v, one rank:
+    scripts/diff_3D/diff_3D_nonlin_multixpu.jl:92-94,123), the others the whole H. The reaction scripts
+    save nothing."""
+    import scipy.io
+
+    spec = SCRIPTS[name]
+    if spec.get("react"):
+        return None
+    nd = spec["ndim"]
+    H = result.H
+    if name.endswith("_multixpu") or name.endswith("_multixpu_perf"):
+        H = H[(slice(1, -1),) * nd]
+    data = {f"H_{nd}D": np.asfortranarray(H)}
+    for d, c in enumerate(("xc", "yc", "zc")[:nd]):
+        data[f"{c}_{nd}D"] = np.asarray(result.coords[d], dtype=np.float64)
+    os.makedirs(out_dir, exist_ok=True)
+    path = os.path.join(out_dir, viz_file_name(name))
+    scipy.io.savemat(path, data, do_compression=True, format="5", oned_as="column")
+    return path
+
+
 def main(argv=None):
     """CLI mirror of running a script: `python -m <pkg>.scripts diff_3D_nonlin_perf` with NX/NY/NZ and
     DO_SAVE from the environment (scripts/diff_3D/diff_3D_nonlin_perf.jl:1-8)."""
@@ -310,6 +343,8 @@ def main(argv=None):
                 f.write(prefix + " ".join(str(v) for v in n) + f" {r.ittot} {r.t_toc} {r.A_eff} {r.t_it} {r.T_eff}\n")
             else:                     # converging rows (scripts/diff_3D/diff_3D_nonlin_multixpu.jl:132)
                 f.write(" ".join(str(v) for v in n) + f" {r.ittot} {len(r.iters)}\n")
+    if _env_bool("DO_SAVE_VIZ", False):
+        save_viz(name, r)
     return 0
 
 

@@ -96,3 +96,44 @@ def test_a_eff_matches_reference_definition(pkg):
     assert pkg.a_eff_gb(3, (512, 512, 512)) == pytest.approx(10.73741824, rel=1e-15)
     # (2*3+2)/1e9*nx*ny*8 at 8192^2 = 4.294967296 GB (output/out_diff_2D_nonlin_perf_ampere.txt:41)
     assert pkg.a_eff_gb(2, (8192, 8192)) == pytest.approx(4.294967296, rel=1e-15)
+
+
+@pytest.mark.parametrize("name,expect_file,inner", [
+    ("diff_1D_lin", "diff_1D_lin.mat", False),
+    ("diff_2D_nonlin_perf", "diff_2D_nonlin_perf.mat", False),
+    ("diff_3D_nonlin_multixpu", "diff_3D_nonlin.mat", True),
+    ("diff_3D_linstep_multixpu", "diff_3D_linstep.mat", True),
+    ("diff_3D_nonlin_perf", "diff_3D_nonlin_perf.mat", False),
+    ("diff_3D_nonlin_multixpu_perf", "diff_3D_nonlin_multixpu_perf.mat", True),
+])
+def test_save_viz_mat_v5(name, expect_file, inner, tmp_path):
+    """DO_SAVE_VIZ dumps (scripts/diff_1D_lin.jl:92, scripts/diff_3D/diff_3D_nonlin_multixpu.jl:137,
+    scripts/diff_3D/diff_3D_nonlin_perf.jl:133): file name, variable names, inn(H) for the multi-xPU
+    scripts, coordinate vectors as n x 1 columns; read back with scipy.io.loadmat."""
+    import scipy.io
+    from importlib import import_module
+    scripts = import_module("ptd_amd.scripts")
+    nd = scripts.SCRIPTS[name]["ndim"]
+    n = [7, 6, 5][:nd]
+    rng = np.random.default_rng(0)
+    H = np.asfortranarray(rng.random(n))
+    coords = tuple(scripts.linrange(0.5 / m, 10 - 0.5 / m, m) for m in n)
+    r = scripts.ScriptResult(name=name, coords=coords, H=H, iters=np.zeros(1, np.int64), errs=np.zeros(1))
+    path = scripts.save_viz(name, r, out_dir=str(tmp_path))
+    assert path.endswith(expect_file)
+    m = scipy.io.loadmat(path)
+    want = H[(slice(1, -1),) * nd] if inner else H
+    got = m[f"H_{nd}D"]
+    assert np.array_equal(got.reshape(want.shape) if nd == 1 else got, want)
+    for d, c in enumerate(("xc", "yc", "zc")[:nd]):
+        v = m[f"{c}_{nd}D"]
+        assert v.shape == (n[d], 1)
+        assert np.array_equal(v[:, 0], coords[d])
+
+
+def test_save_viz_skips_reaction_scripts(tmp_path):
+    from importlib import import_module
+    scripts = import_module("ptd_amd.scripts")
+    r = scripts.ScriptResult(name="diff_1D_react", coords=(np.zeros(4),), H=np.zeros(4),
+                             iters=np.zeros(1, np.int64), errs=np.zeros(1))
+    assert scripts.save_viz("diff_1D_react", r, out_dir=str(tmp_path)) is None
