@@ -47,6 +47,11 @@ int check_params(const ptd_params* p) {
                 "PTD_MIXED runs the fused 3D correction sweeps: ndim must be 3");
     PTD_REQUIRE(p->stop_norm == PTD_NORM_RMS || p->stop_norm == PTD_NORM_MAX, PTD_ERR_ARG, "stop_norm must be PTD_NORM_RMS or PTD_NORM_MAX");
     PTD_REQUIRE(p->arith == PTD_ARITH_EXACT || p->arith == PTD_ARITH_FAST, PTD_ERR_ARG, "arith must be PTD_ARITH_EXACT or PTD_ARITH_FAST");
+    // the FAST / float32 nonlinear Re = pi + a*rsq(a*h3) keeps a*h3 normal at the H^3 floor only for
+    // K = max_l^2/dt >= 1e-6 (the reference's scripts have K ~ 500)
+    PTD_REQUIRE(p->dcase != PTD_NONLIN || (p->arith == PTD_ARITH_EXACT && p->precision == PTD_FP64) ||
+                    p->max_l2 / p->dt >= 1e-6,
+                PTD_ERR_ARG, "FAST / float32 / mixed nonlinear sweeps need max_l^2/dt >= 1e-6");
     PTD_REQUIRE(p->nx >= 3, PTD_ERR_ARG, "nx must be >= 3");
     PTD_REQUIRE(p->ndim < 2 || p->ny >= 3, PTD_ERR_ARG, "ny must be >= 3");
     PTD_REQUIRE(p->ndim < 3 || p->nz >= 3, PTD_ERR_ARG, "nz must be >= 3");

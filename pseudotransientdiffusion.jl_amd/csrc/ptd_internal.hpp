@@ -28,6 +28,8 @@ struct KP {
     T max_l2, resc, d0, theta_r, dtau_rho, re;
     // constants of the FAST arithmetic (rearranged update rules)
     T K;                  // max_l2 * _dt
+    T K8;                 // 8 * K (nonlinear faces work with u = (Ha + Hb)^3 = 8 * Hface^3)
+    T _dx8, _dy8, _dz8;   // _dx / 8, _dy / 8, _dz / 8
     T c1;                 // max_l / Vpdtau * Resc           (theta = c1 / Re)
     T c2;                 // Vpdtau * max_l * Resc           (nonlinear dtau_rho = c2 / (H^3 Re))
     T c3;                 // Vpdtau * max_l / Re             (step dtau_rho = c3 / maxloc(D))
@@ -66,6 +68,10 @@ KP<T> make_kp(const ptd_params& p) {
     k.re = (T)p.re;
     const double _dt = 1.0 / p.dt;
     k.K = (T)(p.max_l2 * _dt);
+    k.K8 = (T)(8.0 * (p.max_l2 * _dt));
+    k._dx8 = (T)(0.125 / p.dx);
+    k._dy8 = (T)(k.hy ? 0.125 / p.dy : 0.0);
+    k._dz8 = (T)(k.hz ? 0.125 / p.dz : 0.0);
     k.c1 = (T)(p.max_l / p.vpdt * p.resc);
     k.c2 = (T)(p.vpdt * p.max_l * p.resc);
     k.c3 = (T)(p.re > 0 ? p.vpdt * p.max_l / p.re : 0.0);
@@ -203,23 +209,28 @@ __device__ __forceinline__ float rsq_1nr(float x) { return __builtin_amdgcn_rsqf
 template <typename T>
 __device__ __forceinline__ T fmaT(T a, T b, T c) { return fma(a, b, c); }
 
-// H^3 floored at the smallest normal number of T: with the reference's initial condition H reaches
-// 1e-31 near the corners and H^3 underflows to 0 in float32 (0 * inf in the Re terms). The floor only
-// acts where H^3 is not representable; there D ~ 0 either way.
-__device__ __forceinline__ float floor_min(float x) { return fmaxf(x, 1.17549435e-38f); }
-__device__ __forceinline__ double floor_min(double x) { return fmax(x, 2.2250738585072014e-308); }
-// Re = pi + sqrt(pi^2 + K/H^3) capped in float32, where K/H^3 overflows for H^3 near the floor: with
-// Re = 1e30, theta = c1/Re ~ 0 and the face flux goes to its D ~ 0 limit instead of 0*inf.
-__device__ __forceinline__ float cap_re(float x) { return fminf(x, 1e30f); }
-__device__ __forceinline__ double cap_re(double x) { return x; }
+// H^3 floored (float32 1e-30, float64 1e-300): with the reference's initial condition H reaches 1e-31
+// near the corners and H^3 underflows to 0 in float32 (0 * inf in the Re terms). The floor only acts
+// where D = H^3 is negligible either way; it keeps a*h3 below normal for K = max_l^2/dt >= 1e-6 (the
+// host rejects smaller K on these paths), so every reciprocal square root below is finite.
+__device__ __forceinline__ float floor_h3(float x) { return fmaxf(x, 1e-30f); }
+__device__ __forceinline__ double floor_h3(double x) { return fmax(x, 1e-300); }
+// the same floor for u = 8*h3
+__device__ __forceinline__ float floor_h3x8(float x) { return fmaxf(x, 8e-30f); }
+__device__ __forceinline__ double floor_h3x8(double x) { return fmax(x, 8e-300); }
 
 // FAST Re = pi + sqrt(pi^2 + K/h3) = pi + a*rsq(a*h3) with a = pi^2*h3 + K: one reciprocal square root
-// instead of a reciprocal and a square root. a*h3 stays finite and normal for h3 in [floor, 1e150]
-// (K = max_l^2/dt > 0).
+// instead of a reciprocal and a square root.
 template <typename T>
 __device__ __forceinline__ T re_fast(const KP<T>& p, T h3) {
     const T a = fmaT(p.pi2, h3, p.K);
-    return cap_re(p.pi + a * rsq_1nr(a * h3));
+    return p.pi + a * rsq_1nr(a * h3);
+}
+// The same from u = 8*h3 (faces: u = (Ha + Hb)^3 needs no 0.5): sqrt(pi^2 + 8K/u) = c*rsq(c*u), c = pi^2*u + 8K.
+template <typename T>
+__device__ __forceinline__ T re_fast8(const KP<T>& p, T u) {
+    const T c = fmaT(p.pi2, u, p.K8);
+    return p.pi + c * rsq_1nr(c * u);
 }
 
 // ---------------------------------------------------------------------------------------------
@@ -248,19 +259,20 @@ struct Phys {
     // New damped flux on the face (a | b) from the old flux q; q2 receives the residual flux
     // (-coef*dH/h, the q2 of the reference) when wanted. Da, Db: aux values (D, or Hc for kDelta).
     template <bool Q2>
-    __device__ static __forceinline__ T face(const KP<T>& p, T q, T Ha, T Hb, Ta Da, Ta Db, T h, T ih, T& q2) {
+    __device__ static __forceinline__ T face(const KP<T>& p, T q, T Ha, T Hb, Ta Da, Ta Db, T h, T ih, T ih8,
+                                             T& q2) {
         const T dH = Hb - Ha;
         if constexpr (DC == kDelta) {
             // correction flux around Hc: g(Hc+delta) - g(Hc) = dD*dHc + (hbar+dbar)^3*ddelta with
-            // dD = (hbar+dbar)^3 - hbar^3 = dbar*(3*hbar*(hbar+dbar) + dbar^2), no cancellation
-            const T hbar = (T)0.5 * (Da + Db);
+            // dD = (hbar+dbar)^3 - hbar^3 = dbar*(3*hbar*(hbar+dbar) + dbar^2), no cancellation. In the
+            // sums S = 2*hbar, Tt = 2*dbar, U = S + Tt: dD = Tt*(3*S*U + Tt^2)/8, (hbar+dbar)^3 = U^3/8.
+            const T S = Da + Db;
+            const T Tt = Ha + Hb;
+            const T U = S + Tt;
+            const T u = floor_h3x8(U * U * U);
             const T dHc = Db - Da;
-            const T dbar = (T)0.5 * (Ha + Hb);
-            const T hsum = hbar + dbar;
-            const T dn = floor_min(hsum * hsum * hsum);
-            const T dD = dbar * fmaT((T)3 * hbar, hsum, dbar * dbar);
-            const T g = fmaT(dD, dHc, dn * dH) * ih;
-            const T re = re_fast(p, dn);
+            const T g = ih8 * fmaT(u, dH, Tt * fmaT((T)3 * S, U, Tt * Tt) * dHc);
+            const T re = re_fast8(p, u);
             if constexpr (Q2) q2 = -g;
             return fmaT(-(q + g), re * rcp_1nr(re + p.c1), q);
         } else if constexpr (ARITH == kExact) {
@@ -271,10 +283,10 @@ struct Phys {
             if constexpr (DC == PTD_NONLIN) {
                 // q' = (q*theta - g)/(1+theta) with theta = c1/Re is q' = q - (q + g)*w, w = Re/(Re + c1):
                 // the fixed point q = -g does not depend on the rounding of w
-                const T hf = (T)0.5 * (Ha + Hb);
-                const T h3 = floor_min(hf * hf * hf);
-                const T re = re_fast(p, h3);
-                const T g = h3 * (ih * dH);
+                const T U = Ha + Hb;  // 2*Hface: u = U^3 = 8*Hface^3, g = u*(ih/8)*dH
+                const T u = floor_h3x8(U * U * U);
+                const T re = re_fast8(p, u);
+                const T g = u * (ih8 * dH);
                 if constexpr (Q2) q2 = -g;
                 return fmaT(-(q + g), re * rcp_1nr(re + p.c1), q);
             } else {
@@ -326,7 +338,7 @@ struct Phys {
             // correction update with dtau from H = Hc + delta; Hold carries R*dt (source R)
             const T src = fmaT(p._dt, Hold, -div);
             const T Hf = Ac + H;
-            const T h3 = floor_min(Hf * Hf * Hf);
+            const T h3 = floor_h3(Hf * Hf * Hf);
             const T P = h3 * re_fast(p, h3);
             return fmaT(fmaT(-H, p._dt, src), p.c2 * rcp_1nr(fmaT(p.c2, p._dt, P)), H);
         } else if constexpr (ARITH == kExact) {
@@ -346,7 +358,7 @@ struct Phys {
                 return fmaT(fmaT(-H, p._dt, src), p.c3 * rcp_1nr(fmaT(p.c3, p._dt, dmax)), H);
             } else {
                 // dtau = c2/(H^3 Re) = c2/P  ==>  v = c2/(P + _dt*c2)
-                const T h3 = floor_min(H * H * H);
+                const T h3 = floor_h3(H * H * H);
                 const T P = h3 * re_fast(p, h3);
                 return fmaT(fmaT(-H, p._dt, src), p.c2 * rcp_1nr(fmaT(p.c2, p._dt, P)), H);
             }

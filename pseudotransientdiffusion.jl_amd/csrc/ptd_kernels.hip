@@ -328,7 +328,7 @@ k_fused3d(const KP<T> p, const T* __restrict__ Hin, T* __restrict__ Hout, const 
         Dkm = ldp(D + psH * (k0 - 1), oA, true);
     }
     T q2zL = 0;
-    T qzL = Ph::template face<CHECK>(p, ldp(qzi + psQz * (k0 - 1), oQz, true), Hkm, Hc, Dkm, Dc, p.dz, p._dz, q2zL);
+    T qzL = Ph::template face<CHECK>(p, ldp(qzi + psQz * (k0 - 1), oQz, true), Hkm, Hc, Dkm, Dc, p.dz, p._dz, p._dz8, q2zL);
     if (upd && k0 == 1) stp(qzo, oQz, qzL);
 
     PlaneOps<T, Ta> A = load_plane(k0);
@@ -350,9 +350,9 @@ k_fused3d(const KP<T> p, const T* __restrict__ Hin, T* __restrict__ Hout, const 
             Dnn = sD[row < R - 1 ? row + 1 : R - 1][lane];
         }
         T q2x = 0, q2y = 0, q2z = 0;
-        const T qxn = Ph::template face<CHECK>(p, A.qx, Hw, Hc, Dw, Dc, p.dx, p._dx, q2x);
-        const T qyn = Ph::template face<CHECK>(p, A.qy, Hs, Hc, Ds, Dc, p.dy, p._dy, q2y);
-        const T qzn = Ph::template face<CHECK>(p, A.qz, Hc, A.Hn, Dc, A.Dn, p.dz, p._dz, q2z);
+        const T qxn = Ph::template face<CHECK>(p, A.qx, Hw, Hc, Dw, Dc, p.dx, p._dx, p._dx8, q2x);
+        const T qyn = Ph::template face<CHECK>(p, A.qy, Hs, Hc, Ds, Dc, p.dy, p._dy, p._dy8, q2y);
+        const T qzn = Ph::template face<CHECK>(p, A.qz, Hc, A.Hn, Dc, A.Dn, p.dz, p._dz, p._dz8, q2z);
         sQ[row][lane] = qyn;
         if constexpr (CHECK) sQ2[row][lane] = q2y;
         __syncthreads();
@@ -538,7 +538,7 @@ k_fused3d_t2(const KP<T> p, const T* __restrict__ Hin, T* __restrict__ Hout, con
     }
     T unused = 0;
     T q1zL = Ph::template face<false>(p, ldp(qzi + psQz * kb, oQz, true), ldp(Hin + psH * kb, oH, true), Hc,
-                                      Dm1, Dc, p.dz, p._dz, unused);
+                                      Dm1, Dc, p.dz, p._dz, p._dz8, unused);
 
     constexpr bool kAhead2 = t2_prefetch_depth<T>() == 2;
     PlaneOps2<T, Ta> A = load_plane(ka);
@@ -577,16 +577,16 @@ k_fused3d_t2(const KP<T> p, const T* __restrict__ Hin, T* __restrict__ Hout, con
             Dpn = sDp[rn][lane];
         }
         T q1x = 0, q1y = 0, q1zU = 0;
-        if (l1y) q1y = Ph::template face<false>(p, A.qy, Hs, Hc, Ds, Dc, p.dy, p._dy, unused);
+        if (l1y) q1y = Ph::template face<false>(p, A.qy, Hs, Hc, Ds, Dc, p.dy, p._dy, p._dy8, unused);
         if (l1all) {
-            q1x = Ph::template face<false>(p, A.qx, Hw, Hc, Dw, Dc, p.dx, p._dx, unused);
-            q1zU = Ph::template face<false>(p, A.qz, Hc, A.Hn, Dc, A.Dn, p.dz, p._dz, unused);
+            q1x = Ph::template face<false>(p, A.qx, Hw, Hc, Dw, Dc, p.dx, p._dx, p._dx8, unused);
+            q1zU = Ph::template face<false>(p, A.qz, Hc, A.Hn, Dc, A.Dn, p.dz, p._dz, p._dz8, unused);
         }
         const T H1w = __shfl_up(H1p, 1);
         const T H1s = sH1[rs][lane];
         T r2x = 0, r2y = 0, q2x = 0, q2y = 0;
-        if (l2y) q2y = Ph::template face<CHECK>(p, q1yp, H1s, H1p, Dps, Dm1, p.dy, p._dy, r2y);
-        if (l2all) q2x = Ph::template face<CHECK>(p, q1xp, H1w, H1p, Dpw, Dm1, p.dx, p._dx, r2x);
+        if (l2y) q2y = Ph::template face<CHECK>(p, q1yp, H1s, H1p, Dps, Dm1, p.dy, p._dy, p._dy8, r2y);
+        if (l2all) q2x = Ph::template face<CHECK>(p, q1xp, H1w, H1p, Dpw, Dm1, p.dx, p._dx, p._dx8, r2x);
         sQ1[row][lane] = q1y;
         sQ2[row][lane] = q2y;
         if constexpr (CHECK) sR2[row][lane] = r2y;
@@ -612,7 +612,7 @@ k_fused3d_t2(const KP<T> p, const T* __restrict__ Hin, T* __restrict__ Hout, con
         }
 
         T r2z = 0, q2zU = 0;
-        if (l2all) q2zU = Ph::template face<CHECK>(p, q1zL, H1p, H1, Dm1, Dc, p.dz, p._dz, r2z);
+        if (l2all) q2zU = Ph::template face<CHECK>(p, q1zL, H1p, H1, Dm1, Dc, p.dz, p._dz, p._dz8, r2z);
         if (k > k0) {
             const int m = k - 1;
             if (l2all) {
