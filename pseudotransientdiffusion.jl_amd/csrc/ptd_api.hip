@@ -251,8 +251,8 @@ int run_delta(ptd_ctx* c, int64_t n, bool check_last) {
 
 // Defect-correction PT solve (PTD_MIXED). Outer loop in float64: R = residual of the backward-Euler
 // system at the iterate Hc (flux from Hc itself); inner loop in float32: PT sweeps on the correction
-// delta with source R (same Re / theta / dtau rules, evaluated at Hc + delta), checked every nout
-// iterations on the float32 correction residual; then Hc += delta. The inner sweeps are the reference
+// delta with source R (same Re / theta / dtau rules, evaluated at Hc + delta), checked on the float32
+// correction residual every nout iterations once a predicted minimum has run; then Hc += delta. The inner sweeps are the reference
 // PT iteration written in shifted variables (delta = H - Hc, p = q + g(Hc)); they start from the true
 // fluxes q carried over from the previous solve, as the reference's q arrays are (starting from the
 // equilibrium flux of a non-equilibrium H instead overshoots). The inner solve stops once it has
@@ -286,6 +286,18 @@ int pt_solve_mixed(ptd_ctx* c, int64_t* iters, double* err_rms, double* err_max)
         double best = HUGE_VAL;
         int stall = 0;
         int64_t it = 0;
+        // A correction solve takes about as many sweeps per decade of reduction as the last one that
+        // met the kInner target: the first check comes after 60% of the predicted count (in whole nout
+        // blocks), then every nout sweeps. (These checks are the float32 inner solve's own; the float64
+        // stopping test above is evaluated after every correction.)
+        const double decades = std::log10(err / target) / std::log10(1.0 / kInner);
+        const int64_t first = (int64_t)(0.6 * (double)c->inner_hint * std::min(1.0, decades)) / p.nout * p.nout;
+        if (first > 0 && total + first < p.itmax) {
+            st = run_delta(c, first, false);
+            if (st != PTD_OK) return st;
+            it = first;
+        }
+        bool met = false;
         while (total + it < p.itmax) {
             const int64_t n = std::min<int64_t>(p.nout, p.itmax - total - it);
             st = run_delta(c, n, true);
@@ -295,7 +307,11 @@ int pt_solve_mixed(ptd_ctx* c, int64_t* iters, double* err_rms, double* err_max)
             st = fetch_errors(c, &irms, &imx);
             if (st != PTD_OK) return st;
             const double r = p.stop_norm == PTD_NORM_MAX ? imx : irms;
-            if (std::isnan(r) || r <= target) break;
+            if (std::isnan(r)) break;
+            if (r <= target) {
+                met = target > 0.5 * p.tol;
+                break;
+            }
             if (r < 0.95 * best) {
                 best = r;
                 stall = 0;
@@ -304,6 +320,7 @@ int pt_solve_mixed(ptd_ctx* c, int64_t* iters, double* err_rms, double* err_max)
             }
         }
         total += it;
+        if (met) c->inner_hint = it;
         PTD_HIP(ptd::launch_flux_state(c, false));  // q = p - g(Hc), before Hc moves
         PTD_HIP(ptd::launch_add_correction(c));
         st = residual64(&rms, &mx);

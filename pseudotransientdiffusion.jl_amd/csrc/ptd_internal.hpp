@@ -406,6 +406,7 @@ struct ptd_ctx {
     void* hcf;                  // float copy of the float64 iterate (aux of the D = H^3 correction sweep)
     int dcur, pcur;
     int64_t outer_last;         // outer (defect-correction) iterations of the last PT solve
+    int64_t inner_hint;         // float32 sweeps of the last correction solve that met its reduction target
 };
 
 namespace ptd {
