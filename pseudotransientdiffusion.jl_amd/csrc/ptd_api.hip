@@ -219,7 +219,7 @@ int run_iterations(ptd_ctx* c, int64_t n, bool check_last) {
     const char* t2e = getenv("PTD_T2");
     const bool t2_env = !(t2e && t2e[0] == '0');
     const bool plain = c->p.form == PTD_PLAIN;
-    const bool t2 = t2_env && ptd::fused_available(c);
+    const bool t2 = t2_env && ptd::fused2_available(c);
     while (n > 0) {
         if (t2 && n >= 2) {
             const bool chk = check_last && n == 2;

@@ -419,6 +419,7 @@ hipError_t launch_residual(ptd_ctx* c, bool write_resh);  // partial sums -> dre
 hipError_t launch_fused(ptd_ctx* c, bool check);          // one fused PT iteration (3D)
 hipError_t launch_fused2(ptd_ctx* c, bool check);         // two fused PT iterations in one pass (3D)
 bool fused_available(const ptd_ctx* c);
+bool fused2_available(const ptd_ctx* c);
 // mixed precision: float32 correction sweeps (1 or 2 iterations), source and correction kernels
 hipError_t launch_delta_sweep(ptd_ctx* c, bool two, bool check);
 hipError_t launch_make_source(ptd_ctx* c);      // src = float(ResH * dt) on inner cells

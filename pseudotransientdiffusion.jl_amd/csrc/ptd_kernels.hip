@@ -397,6 +397,109 @@ k_fused3d(const KP<T> p, const T* __restrict__ Hin, T* __restrict__ Hout, const 
 }
 
 // ---------------------------------------------------------------------------------------------
+// fused 2D PT iteration
+// ---------------------------------------------------------------------------------------------
+//
+// The 3D sweep's tile on a single plane: block = 64 lanes (x) x R rows (y); lane L of tile bx holds
+// i = 62*bx + L and row r of tile by holds j = (R-2)*by + r. Each thread computes the flux on its lower
+// x face (shuffle for the upper one) and lower y face (LDS row+1 for the upper one), then updates its
+// cell (lanes 1..62, rows 1..R-2). Same expressions, same ownership of every face and cell as the 3D
+// sweep, so EXACT arithmetic reproduces compute_flux! + compute_update! bit for bit. CHECK also
+// evaluates the plain form's residual (residual fluxes from the pre-update H, transient term from the
+// updated H: scripts/diff_2D_nonlin.jl:28-37) and reduces it per tile.
+template <typename T, int DC, int FORM, int ARITH, int R, bool CHECK>
+__device__ __forceinline__ void fused2d_tile(const KP<T>& p, const T* __restrict__ Hin, T* __restrict__ Hout,
+                                             const T* __restrict__ Hold, const T* __restrict__ D,
+                                             const T* __restrict__ qxi, const T* __restrict__ qyi,
+                                             T* __restrict__ qxo, T* __restrict__ qyo,
+                                             double2* __restrict__ part, int bx, int by) {
+    using Ph = Phys<T, DC, FORM, ARITH>;
+    constexpr bool STEP = DC == PTD_STEP;
+    __shared__ T sH[R][64];
+    __shared__ T sQ[R][64];
+    __shared__ T sD[STEP ? R : 1][64];
+    __shared__ T sQ2[CHECK ? R : 1][64];
+
+    const int lane = threadIdx.x, row = threadIdx.y;
+    const int nx = (int)p.nx, ny = (int)p.ny;
+    const int i = 62 * bx + lane;
+    const int j = (R - 2) * by + row;
+    const bool xin = (i >= 1) && (i <= nx - 2);
+    const bool yin = (j >= 1) && (j <= ny - 2);
+    const bool rmid = (row >= 1) && (row <= R - 2);
+    const bool lmid = (lane >= 1) && (lane <= 62);
+    const bool upd = lmid && rmid && xin && yin;
+    const bool fx = (lane >= 1) && (i <= nx - 1) && rmid && yin;
+    const bool fy = (row >= 1) && (j <= ny - 1) && lmid && xin;
+    const bool wx = fx && ((lane <= 62 && i <= nx - 2) || i == nx - 1);
+    const bool wy = fy && ((row <= R - 2 && j <= ny - 2) || j == ny - 1);
+
+    // clamped indices: every load is in bounds; values outside the computed set are never stored
+    const int ic = min(i, nx - 1), jc = min(j, ny - 1);
+    const int ixq = max(1, min(i, nx - 1)), jxq = max(1, min(j, ny - 2));
+    const int iyq = max(1, min(i, nx - 2)), jyq = max(1, min(j, ny - 1));
+    const int64_t oH = (int64_t)ic + (int64_t)nx * jc;
+    const int64_t oQx = (int64_t)(ixq - 1) + (int64_t)(nx - 1) * (jxq - 1);
+    const int64_t oQy = (int64_t)(iyq - 1) + (int64_t)(nx - 2) * (jyq - 1);
+
+    const T Hc = Hin[oH];
+    const T Ho = Hold[oH];
+    T Dc = 0;
+    if constexpr (STEP) Dc = D[oH];
+    sH[row][lane] = Hc;
+    if constexpr (STEP) sD[row][lane] = Dc;
+    __syncthreads();
+
+    const T Hw = __shfl_up(Hc, 1);
+    const T Hs = sH[row > 0 ? row - 1 : 0][lane];
+    T Dw = 0, De = 0, Ds = 0, Dn = 0;
+    if constexpr (STEP) {
+        Dw = __shfl_up(Dc, 1);
+        De = __shfl_down(Dc, 1);
+        Ds = sD[row > 0 ? row - 1 : 0][lane];
+        Dn = sD[row < R - 1 ? row + 1 : R - 1][lane];
+    }
+    T q2x = 0, q2y = 0;
+    const T qxn = Ph::template face<CHECK>(p, qxi[oQx], Hw, Hc, Dw, Dc, p.dx, p._dx, p._dx8, q2x);
+    const T qyn = Ph::template face<CHECK>(p, qyi[oQy], Hs, Hc, Ds, Dc, p.dy, p._dy, p._dy8, q2y);
+    sQ[row][lane] = qyn;
+    if constexpr (CHECK) sQ2[row][lane] = q2y;
+    __syncthreads();
+
+    const T qxE = __shfl_down(qxn, 1);
+    const T qyN = sQ[row < R - 1 ? row + 1 : R - 1][lane];
+    T dmax = 0;
+    if constexpr (STEP) {
+        dmax = fmax(fmax(Dw, De), Dc);
+        dmax = fmax(dmax, fmax(Ds, Dn));
+    }
+    const T dv = Ph::template div<2>(p, qxE - qxn, qyN - qyn, (T)0);
+    const T Hnew = Ph::cell(p, Hc, Ho, dv, dmax, Dc);
+    if (upd) Hout[oH] = Hnew;
+    if (wx) qxo[oQx] = qxn;
+    if (wy) qyo[oQy] = qyn;
+    if constexpr (CHECK) {
+        const T q2xE = __shfl_down(q2x, 1);
+        const T q2yN = sQ2[row < R - 1 ? row + 1 : R - 1][lane];
+        const T dv2 = Ph::template div<2>(p, q2xE - q2x, q2yN - q2y, (T)0);
+        const double r = upd ? (double)Ph::resid(p, Hnew, Ho, dv2) : 0.0;
+        block_sum_max(r * r, upd ? fabs(r) : 0.0, row * 64 + lane, 64 * R, part);
+    }
+}
+
+template <typename T, int DC, int FORM, int ARITH, int R, bool CHECK>
+__global__ void __launch_bounds__(64 * R)
+k_fused2d(const KP<T> p, const T* __restrict__ Hin, T* __restrict__ Hout, const T* __restrict__ Hold,
+          const T* __restrict__ D, const T* __restrict__ qxi, const T* __restrict__ qyi, T* __restrict__ qxo,
+          T* __restrict__ qyo, double2* __restrict__ partial, int ntx, int ntiles, int per_xcd) {
+    const int b = blockIdx.x;
+    const int tile = (b & 7) * per_xcd + (b >> 3);  // XCD-aware: each XCD walks a contiguous tile range
+    if (tile >= ntiles) return;
+    fused2d_tile<T, DC, FORM, ARITH, R, CHECK>(p, Hin, Hout, Hold, D, qxi, qyi, qxo, qyo, &partial[tile],
+                                               tile % ntx, tile / ntx);
+}
+
+// ---------------------------------------------------------------------------------------------
 // fused 3D sweep over two PT iterations (temporal blocking)
 // ---------------------------------------------------------------------------------------------
 //
@@ -804,6 +907,40 @@ static hipError_t sweep_run(ptd_ctx* c, const SweepBufs& b, bool two, bool check
     return hipGetLastError();
 }
 
+static constexpr int kRows2D = 16;  // rows (waves) per block of the 2D sweep
+
+// One fused 2D sweep (one iteration); with `check` the plain-form residual is reduced into c->dred.
+template <typename T, int DC, int FORM, int ARITH>
+static hipError_t sweep2d_run(ptd_ctx* c, const SweepBufs& b, bool check) {
+    const KP<T> kp = make_kp<T>(c->p);
+    const int ntx = (int)((c->p.nx - 2 + 61) / 62);
+    const int nty = (int)((c->p.ny - 2 + kRows2D - 3) / (kRows2D - 2));
+    const int ntiles = ntx * nty;
+    const int per = (ntiles + 7) / 8;
+    const dim3 g(8 * per), bl(64, kRows2D);
+    const T* Hin = (const T*)b.Hin;
+    T* Hout = (T*)b.Hout;
+    const T* Ho = (const T*)b.Hold;
+    const T* A = (const T*)b.aux;
+    const T* q0 = (const T*)b.qin[0];
+    const T* q1 = (const T*)b.qin[1];
+    T* o0 = (T*)b.qout[0];
+    T* o1 = (T*)b.qout[1];
+    double2* part = (double2*)c->partial;
+    if (check)
+        hipLaunchKernelGGL((k_fused2d<T, DC, FORM, ARITH, kRows2D, true>), g, bl, 0, c->stream, kp, Hin, Hout, Ho, A,
+                           q0, q1, o0, o1, part, ntx, ntiles, per);
+    else
+        hipLaunchKernelGGL((k_fused2d<T, DC, FORM, ARITH, kRows2D, false>), g, bl, 0, c->stream, kp, Hin, Hout, Ho,
+                           A, q0, q1, o0, o1, part, ntx, ntiles, per);
+    hipError_t e = hipGetLastError();
+    if (e != hipSuccess) return e;
+    if (check)
+        hipLaunchKernelGGL(k_reduce_partials, dim3(1), dim3(kBlock), 0, c->stream, (const double2*)c->partial,
+                           (int64_t)ntiles, c->dred);
+    return hipGetLastError();
+}
+
 // The context's own fields (H, q double-buffered; the sweep flips them).
 static SweepBufs ctx_bufs(ptd_ctx* c) {
     const int h = c->hcur, q = c->qcur;
@@ -822,8 +959,13 @@ static SweepBufs ctx_bufs(ptd_ctx* c) {
 template <typename T, int DC, int FORM, bool TWO>
 static hipError_t ctx_sweep_t(ptd_ctx* c, bool check) {
     const SweepBufs b = ctx_bufs(c);
-    hipError_t e = c->p.arith == PTD_ARITH_FAST ? sweep_run<T, DC, FORM, kFast>(c, b, TWO, check)
-                                                : sweep_run<T, DC, FORM, kExact>(c, b, TWO, check);
+    hipError_t e;
+    if (c->p.ndim == 2)
+        e = c->p.arith == PTD_ARITH_FAST ? sweep2d_run<T, DC, FORM, kFast>(c, b, check)
+                                         : sweep2d_run<T, DC, FORM, kExact>(c, b, check);
+    else
+        e = c->p.arith == PTD_ARITH_FAST ? sweep_run<T, DC, FORM, kFast>(c, b, TWO, check)
+                                         : sweep_run<T, DC, FORM, kExact>(c, b, TWO, check);
     if (e != hipSuccess) return e;
     c->hcur = 1 - c->hcur;
     c->qcur = 1 - c->qcur;
@@ -880,8 +1022,10 @@ hipError_t launch_iter_params(ptd_ctx* c) {
 }
 
 bool fused_available(const ptd_ctx* c) {
-    return c->p.ndim == 3 && c->p.nx >= 3 && c->p.ny >= 3 && c->p.nz >= 3;
+    return (c->p.ndim == 3 && c->p.nx >= 3 && c->p.ny >= 3 && c->p.nz >= 3) ||
+           (c->p.ndim == 2 && c->p.nx >= 3 && c->p.ny >= 3);
 }
+bool fused2_available(const ptd_ctx* c) { return c->p.ndim == 3 && fused_available(c); }
 
 // ---------------------------------------------------------------------------------------------
 // mixed precision: float32 correction sweeps around a float64 iterate
@@ -1028,6 +1172,10 @@ hipError_t launch_delta_sweep(ptd_ctx* c, bool two, bool check) {
 // Number of (sum,max) partials the reductions can produce for this context.
 int64_t partial_capacity(const ptd_params& p, int64_t nR) {
     int64_t nb = (nR + kBlock - 1) / kBlock;
+    if (p.ndim == 2) {
+        const int64_t ntx = (p.nx - 2 + 61) / 62, nty = (p.ny - 2 + kRows2D - 3) / (kRows2D - 2);
+        nb = std::max<int64_t>(nb, ntx * nty);
+    }
     if (p.ndim == 3) {
         const int64_t ntx = (p.nx - 2 + 61) / 62, nty = (p.ny - 2 + kRows - 3) / (kRows - 2);
         // smallest chunk is one plane
