@@ -74,6 +74,35 @@ def solve_steps(ctx, nt, tag):
     return iters, errs
 
 
+def batch_leg(pkg, count, seed=0):
+    """The batched workload: `count` independent float64 solves (FAST arithmetic) in one ptd_batch.
+    Aggregate T_eff = sum over members of A_eff(member) * PT iterations(member) / wall time."""
+    B = pkg.batch
+    w = B.workload(count=count, seed=seed)
+    params = [B.member_params(m) for m in w]
+    log(f"batch: {count} members, building fields")
+    with pkg.Batch(params) as b:
+        for i, m in enumerate(w):
+            H0, D = B.member_fields(m)
+            b.set(i, "H", H0)
+            b.set(i, "Hold", H0)
+            if D is not None:
+                b.set(i, "D", D)
+        log(f"batch: solving ({b.device_bytes / 1e9:.1f} GB on the device)")
+        it, er, lat, st, total_ms = b.solve(5)
+        tz, rows, cand_ms = b.tiling()
+    gb = sum(B.member_a_eff_gb(m) * int(it[i].sum()) for i, m in enumerate(w))
+    cells = sum(m.n ** (2 if "2D" in m.script else 3) for m in w)
+    out = {"members": count, "seed": seed, "cells": int(cells), "physical_steps": 5,
+           "wall_s": round(total_ms / 1e3, 4), "T_eff_aggregate_GBs": round(gb / (total_ms / 1e3), 2),
+           "latency_ms_p50": round(B.percentile(lat, 50), 2), "latency_ms_p99": round(B.percentile(lat, 99), 2),
+           "pt_iterations_total": int(it.sum()), "all_converged": bool((st == 0).all() and (er <= 1e-8).all()),
+           "tiling": {"tz3d": tz, "rows2d": rows, "candidate_block_ms": [round(v, 3) for v in cand_ms]}}
+    log(f"batch: {out['wall_s']} s, aggregate T_eff {out['T_eff_aggregate_GBs']} GB/s, "
+        f"latency p50 {out['latency_ms_p50']} ms p99 {out['latency_ms_p99']} ms")
+    return out
+
+
 def cpu_baseline(pkg, n, seconds):
     """The oracle's PT iterations (OpenMP C restatement of the reference Threads path) at n^3 on this
     host: run iterations until `seconds` have elapsed (at least one), report T_eff."""
@@ -120,6 +149,9 @@ def main():
     ap.add_argument("--cpu-seconds", type=float, default=15.0)
     ap.add_argument("--tts-n", type=int, default=0,
                     help="grid size of the time-to-solution legs (default: --n; the precision workload is 1023)")
+    ap.add_argument("--batch", type=int, default=0,
+                    help="also solve the batched workload with this many members (96: 2D {128..1024}^2 and "
+                         "3D {64..256}^3 grids, lin / step / H^3, 5 physical steps each)")
     ap.add_argument("--calib-copy", action="store_true",
                     help="only run the 1 GiB copy used to scale rocprofv3 FETCH_SIZE/WRITE_SIZE, then exit")
     args = ap.parse_args()
@@ -259,6 +291,9 @@ def main():
                                         "mixed_final_err": float(merrs[-1]), "speedup": round(tts / tts_m, 3),
                                         "mixed_vs_fp64_rel_maxnorm": err_m})
         log(f"mixed {tts_m:.3f} s ({tts / tts_m:.2f}x), sweeps {list(miters)}, max-norm vs FP64 {err_m:.2e}")
+
+    if args.batch:
+        out["batch"] = batch_leg(pkg, args.batch)
 
     if not args.no_cpu:
         log(f"CPU baseline (~{args.cpu_seconds:.0f} s)")

@@ -177,6 +177,37 @@ int32_t ptd_pt_solve(ptd_ctx* ctx, int64_t* iters, double* err_rms, double* err_
  * iters[nt] and errs[nt] (may be NULL) receive per-step iteration counts and final errors. */
 int32_t ptd_solve(ptd_ctx* ctx, int32_t nt, int64_t* iters, double* errs);
 
+/* ---- batches of independent solves ------------------------------------------------------------
+ *
+ * No reference counterpart: the reference solves one problem per script run. A batch holds n
+ * independent float64 problems (2D or 3D, lin / step / nonlin, each with its own grid and scalars,
+ * the converging scripts' PTD_PLAIN form, one nout for all, itmax a multiple of nout, nout even) and
+ * solves them together: one launch per (ndim, dcase, arith) group covers every member's tiles, each
+ * member's PT loop (checks, stopping, Hold .= H, physical steps) runs on the device, and one check
+ * block is a replayed HIP graph. Every member's result, iteration counts included, is bit-identical to
+ * ptd_solve on that problem alone. */
+typedef struct ptd_batch ptd_batch;
+
+/* params[n], 1 <= n <= 1024. Allocates every member's fields (zeroed) on `device`. */
+int32_t ptd_batch_create(const ptd_params* params, int32_t n, int32_t device, ptd_batch** out);
+int32_t ptd_batch_destroy(ptd_batch* batch);
+int32_t ptd_batch_size(const ptd_batch* batch);
+int64_t ptd_batch_device_bytes(const ptd_batch* batch);
+/* Fields of one member, reference layout as ptd_set_field / ptd_get_field: H (set writes both
+ * buffers), HOLD and D (step case) can be set; H, HOLD, D, QX..QZ can be read. */
+int32_t ptd_batch_set_field(ptd_batch* batch, int32_t member, int32_t field, const double* host, int64_t count);
+int32_t ptd_batch_get_field(ptd_batch* batch, int32_t member, int32_t field, double* host, int64_t count);
+/* nt physical steps of every member (ptd_solve of each, together). iters[n*nt] / errs[n*nt] receive
+ * per-member, per-step iteration counts and final errors (member-major), latency_ms[n] the time from
+ * the start of the call until each member finished, status[n] PTD_OK or PTD_ERR_NAN, *total_ms the
+ * wall time of the call; any output may be NULL. The first call also picks the tiling of the
+ * non-checking launches by timing one check block with each candidate (results do not depend on it). */
+int32_t ptd_batch_solve(ptd_batch* batch, int32_t nt, int64_t* iters, double* errs, double* latency_ms,
+                        int32_t* status, double* total_ms);
+/* The tiling chosen by the first solve: z chunk depth of the 3D launches and rows per block of the 2D
+ * launches, and the measured time of one check block per candidate (PTD_ERR_STATE before a solve). */
+int32_t ptd_batch_tiling(const ptd_batch* batch, int32_t* tz3d, int32_t* rows2d, double* candidate_ms, int32_t ncand);
+
 #ifdef __cplusplus
 }
 #endif

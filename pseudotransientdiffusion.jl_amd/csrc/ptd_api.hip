@@ -340,6 +340,12 @@ int pt_solve_mixed(ptd_ctx* c, int64_t* iters, double* err_rms, double* err_max)
 
 }  // namespace
 
+namespace ptd {
+// shared with the batch entry points (ptd_batch.hip)
+int validate_params(const ptd_params* p) { return check_params(p); }
+int set_error(int code, const std::string& msg) { return fail(code, msg); }
+}  // namespace ptd
+
 extern "C" {
 
 int32_t ptd_abi_version(void) { return PTD_ABI_VERSION; }

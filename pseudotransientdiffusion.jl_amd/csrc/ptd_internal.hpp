@@ -6,6 +6,8 @@
 #pragma once
 
 #include <hip/hip_runtime.h>
+
+#include <string>
 #include <stdint.h>
 
 #include "../../include/ptd.h"
@@ -416,7 +418,7 @@ hipError_t launch_flux(ptd_ctx* c, int q2mode);       // 0: q only, 1: q and q2,
 hipError_t launch_update(ptd_ctx* c);
 hipError_t launch_iter_params(ptd_ctx* c);
 hipError_t launch_residual(ptd_ctx* c, bool write_resh);  // partial sums -> dred
-hipError_t launch_fused(ptd_ctx* c, bool check);          // one fused PT iteration (3D)
+hipError_t launch_fused(ptd_ctx* c, bool check);          // one fused PT iteration (2D / 3D)
 hipError_t launch_fused2(ptd_ctx* c, bool check);         // two fused PT iterations in one pass (3D)
 bool fused_available(const ptd_ctx* c);
 bool fused2_available(const ptd_ctx* c);
@@ -428,5 +430,54 @@ hipError_t launch_f64_to_f32(ptd_ctx* c, float* dst, const double* src, int64_t 
 // correction fluxes <-> true fluxes: to_p: p = float(q + g(H)); !to_p: q = double(p) - g(H), where g(H)
 // is the equilibrium flux magnitude coef*dH/h of the float64 iterate H on each face
 hipError_t launch_flux_state(ptd_ctx* c, bool to_p);
+
+// ---- batches of independent float64 solves (ptd_batch.hip) -----------------------------------
+
+// Device-resident state of one member of a batch. The sweep kernels read the member's fields and
+// buffer parities from it; the controller kernel advances its PT loop after every check block.
+struct BatchMember {
+    KP<double> p;
+    double* H[2];
+    double* Hold;
+    const double* D;
+    double* q[3][2];
+    double2* partial;     // per-tile residual partials of the checking launch
+    int64_t nH;           // cells
+    int64_t iter;         // PT iterations of the current physical step
+    int64_t itmax;
+    int32_t ndim, dcase, arith, nout;
+    int32_t lpb;          // sweep launches per check block (3D: nout / 2 two-level launches; 2D: nout)
+    int32_t active;       // 0 once every physical step is done (or the error is NaN)
+    int32_t hpar, qpar;   // H / q buffer holding the current iterate at the start of a check block
+    int32_t assign;       // set by the controller: Hold <- H before the next block (the step ended)
+    int32_t status;       // PTD_OK or PTD_ERR_NAN
+    int32_t step, nt;     // physical steps done / requested
+    int32_t stop_norm, ntiles_check;
+    double tol, norm_len;
+    double2 red;          // (sum of squares, max) of the last check
+    uint64_t t_done;      // device wall clock when the member finished
+};
+
+// One block of a batched sweep launch: the member it works on, its tile and z chunk depth.
+struct BatchTile {
+    int32_t member, tile, tz, pad;
+};
+
+enum { kBatchMaxMembers = 1024 };
+
+int validate_params(const ptd_params* p);          // the single-problem checks (ptd_api.hip)
+int set_error(int code, const std::string& msg);   // sets ptd_last_error(), returns code
+
+// A batched sweep launch over the tiles of every member of one (ndim, dcase, arith) group: launch
+// `li` of the check block (buffers selected by the members' parities + li), with rows R per block.
+hipError_t launch_batch_sweep(hipStream_t s, int ndim, int dcase, int arith, int rows, bool check,
+                              BatchMember* mem, const BatchTile* tiles, int ntiles, int li);
+int batch_rows_nocheck(int ndim, int dcase, int arith, int variant);  // rows of the non-checking launch
+int batch_rows_check(int ndim);                                       // rows of the checking launch
+hipError_t launch_batch_reduce(hipStream_t s, BatchMember* mem, const int32_t* members, int n);
+hipError_t launch_batch_control(hipStream_t s, BatchMember* mem, int n, int64_t* iters, double* errs,
+                                int32_t* active_count, int nt_cap);
+hipError_t launch_batch_assign(hipStream_t s, BatchMember* mem, int n);
+hipError_t launch_batch_clock(hipStream_t s, uint64_t* t);
 
 }  // namespace ptd

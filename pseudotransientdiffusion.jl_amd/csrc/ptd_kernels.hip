@@ -188,8 +188,7 @@ k_iter_params(const KP<T> p, const T* __restrict__ D, T* __restrict__ dtau, int6
 }
 
 // Sum of the per-block partials in a fixed order (deterministic): out = (sum, max).
-__global__ void __launch_bounds__(kBlock)
-k_reduce_partials(const double2* __restrict__ partial, int64_t n, double* __restrict__ out) {
+__device__ __forceinline__ double2 reduce_partials_block(const double2* __restrict__ partial, int64_t n) {
     double s = 0.0, m = 0.0;
     for (int64_t t = threadIdx.x; t < n; t += kBlock) {
         const double2 v = partial[t];
@@ -199,6 +198,12 @@ k_reduce_partials(const double2* __restrict__ partial, int64_t n, double* __rest
     __shared__ double2 res;
     block_sum_max(s, m, threadIdx.x, kBlock, &res);
     __syncthreads();
+    return res;
+}
+
+__global__ void __launch_bounds__(kBlock)
+k_reduce_partials(const double2* __restrict__ partial, int64_t n, double* __restrict__ out) {
+    const double2 res = reduce_partials_block(partial, n);
     if (threadIdx.x == 0) {
         out[0] = res.x;
         out[1] = res.y;
@@ -545,12 +550,16 @@ constexpr int t2_rows() {
 }
 constexpr int kMinRows2 = 12;  // smallest two-level block (sizes the partials buffer)
 
+// One tile (bx, by) x z chunk bz of the two-level sweep; `part` receives the tile's residual partial
+// (CHECK). Called by the single-problem kernel below and by the batched kernel.
 template <typename T, int DC, int FORM, int ARITH, int R, bool CHECK>
-__global__ void __launch_bounds__(64 * R, (t2_waves_per_simd<T, DC, ARITH, R, CHECK>()))
-k_fused3d_t2(const KP<T> p, const T* __restrict__ Hin, T* __restrict__ Hout, const T* __restrict__ Hold,
-             const typename AuxOf<T, DC>::type* __restrict__ D, const T* __restrict__ qxi, const T* __restrict__ qyi,
-             const T* __restrict__ qzi, T* __restrict__ qxo, T* __restrict__ qyo, T* __restrict__ qzo,
-             double2* __restrict__ partial, int tz, int ntx, int nty, int ntiles, int per_xcd) {
+__device__ __forceinline__ void fused3d_t2_tile(const KP<T>& p, const T* __restrict__ Hin, T* __restrict__ Hout,
+                                                const T* __restrict__ Hold,
+                                                const typename AuxOf<T, DC>::type* __restrict__ D,
+                                                const T* __restrict__ qxi, const T* __restrict__ qyi,
+                                                const T* __restrict__ qzi, T* __restrict__ qxo,
+                                                T* __restrict__ qyo, T* __restrict__ qzo,
+                                                double2* __restrict__ part, int bx, int by, int bz, int tz) {
     using Ph = Phys<T, DC, FORM, ARITH>;
     using Ta = typename AuxOf<T, DC>::type;
     constexpr bool STEP = DC == PTD_STEP;
@@ -562,14 +571,6 @@ k_fused3d_t2(const KP<T> p, const T* __restrict__ Hin, T* __restrict__ Hout, con
     __shared__ Ta sD[AUX ? R : 1][64];           // aux at plane k
     __shared__ Ta sDp[AUX ? R : 1][64];          // aux at plane k-1
     __shared__ T sR2[CHECK ? R : 1][64];         // level-2 residual y flux
-
-    const int b = blockIdx.x;
-    const int tile = (b & 7) * per_xcd + (b >> 3);
-    if (tile >= ntiles) return;
-    const int bx = tile % ntx;
-    const int rest = tile / ntx;
-    const int by = rest % nty;
-    const int bz = rest / nty;
 
     const int lane = threadIdx.x, row = threadIdx.y;
     const int nx = (int)p.nx, ny = (int)p.ny, nz = (int)p.nz;
@@ -762,7 +763,22 @@ k_fused3d_t2(const KP<T> p, const T* __restrict__ Hin, T* __restrict__ Hout, con
             A = load_plane(k + 1 <= nz - 1 ? k + 1 : nz - 1);
         }
     }
-    if constexpr (CHECK) block_sum_max((double)acc, (double)amax, row * 64 + lane, 64 * R, &partial[tile]);
+    if constexpr (CHECK) block_sum_max((double)acc, (double)amax, row * 64 + lane, 64 * R, part);
+}
+
+template <typename T, int DC, int FORM, int ARITH, int R, bool CHECK>
+__global__ void __launch_bounds__(64 * R, (t2_waves_per_simd<T, DC, ARITH, R, CHECK>()))
+k_fused3d_t2(const KP<T> p, const T* __restrict__ Hin, T* __restrict__ Hout, const T* __restrict__ Hold,
+             const typename AuxOf<T, DC>::type* __restrict__ D, const T* __restrict__ qxi, const T* __restrict__ qyi,
+             const T* __restrict__ qzi, T* __restrict__ qxo, T* __restrict__ qyo, T* __restrict__ qzo,
+             double2* __restrict__ partial, int tz, int ntx, int nty, int ntiles, int per_xcd) {
+    const int b = blockIdx.x;
+    const int tile = (b & 7) * per_xcd + (b >> 3);  // XCD-aware: each XCD walks a contiguous tile range
+    if (tile >= ntiles) return;
+    const int bx = tile % ntx;
+    const int rest = tile / ntx;
+    fused3d_t2_tile<T, DC, FORM, ARITH, R, CHECK>(p, Hin, Hout, Hold, D, qxi, qyi, qzi, qxo, qyo, qzo,
+                                                  &partial[tile], bx, rest % nty, rest / nty, tz);
 }
 
 // ---------------------------------------------------------------------------------------------
@@ -1184,6 +1200,204 @@ int64_t partial_capacity(const ptd_params& p, int64_t nR) {
         nb = std::max<int64_t>(nb, ntx2 * nty2 * (p.nz - 2));
     }
     return std::max<int64_t>(nb, 1);
+}
+
+// ---------------------------------------------------------------------------------------------
+// batches of independent float64 solves
+// ---------------------------------------------------------------------------------------------
+//
+// One launch covers the tiles of every member of a (ndim, dcase, arith) group: block b takes table
+// entry (b mod 8)*per + b/8 (XCD-aware), i.e. (member, tile, z chunk depth), and runs the same tile
+// function as the single-problem sweep on that member's fields. Buffers come from the member's
+// parities + the launch index li in the check block; members that are done exit at once.
+
+template <int DC, int ARITH, int R, bool CHECK>
+__global__ void __launch_bounds__(64 * R, (t2_waves_per_simd<double, DC, ARITH, R, CHECK>()))
+k_batch3d_t2(BatchMember* __restrict__ mem, const BatchTile* __restrict__ tiles, int ntot, int per_xcd, int li) {
+    const int b = blockIdx.x;
+    const int t = (b & 7) * per_xcd + (b >> 3);
+    if (t >= ntot) return;
+    const BatchTile e = tiles[t];
+    const int mi = __builtin_amdgcn_readfirstlane(e.member);
+    const int tile = __builtin_amdgcn_readfirstlane(e.tile);
+    const int tz = __builtin_amdgcn_readfirstlane(e.tz);
+    const BatchMember& M = mem[mi];
+    if (!M.active) return;
+    const int ntx = ((int)M.p.nx - 2 + 59) / 60;
+    const int nty = ((int)M.p.ny - 2 + R - 5) / (R - 4);
+    const int hp = (M.hpar + li) & 1, qp = (M.qpar + li) & 1;
+    const int rest = tile / ntx;
+    fused3d_t2_tile<double, DC, PTD_PLAIN, ARITH, R, CHECK>(
+        M.p, M.H[hp], M.H[hp ^ 1], M.Hold, M.D, M.q[0][qp], M.q[1][qp], M.q[2][qp], M.q[0][qp ^ 1],
+        M.q[1][qp ^ 1], M.q[2][qp ^ 1], M.partial + tile, tile % ntx, rest % nty, rest / nty, tz);
+}
+
+template <int DC, int ARITH, int R, bool CHECK>
+__global__ void __launch_bounds__(64 * R)
+k_batch2d(BatchMember* __restrict__ mem, const BatchTile* __restrict__ tiles, int ntot, int per_xcd, int li) {
+    const int b = blockIdx.x;
+    const int t = (b & 7) * per_xcd + (b >> 3);
+    if (t >= ntot) return;
+    const BatchTile e = tiles[t];
+    const int mi = __builtin_amdgcn_readfirstlane(e.member);
+    const int tile = __builtin_amdgcn_readfirstlane(e.tile);
+    const BatchMember& M = mem[mi];
+    if (!M.active) return;
+    const int ntx = ((int)M.p.nx - 2 + 61) / 62;
+    const int hp = (M.hpar + li) & 1, qp = (M.qpar + li) & 1;
+    fused2d_tile<double, DC, PTD_PLAIN, ARITH, R, CHECK>(M.p, M.H[hp], M.H[hp ^ 1], M.Hold, M.D, M.q[0][qp],
+                                                         M.q[1][qp], M.q[0][qp ^ 1], M.q[1][qp ^ 1],
+                                                         M.partial + tile, tile % ntx, tile / ntx);
+}
+
+// rows of the batched launches: the checking launch uses the single-problem checking tiling (so the
+// residual partials, and hence every stopping decision, are those of a single-problem solve); the
+// other launches may use any tiling (every cell is computed by its owner with the same expressions).
+int batch_rows_check(int ndim) { return ndim == 3 ? 16 : kRows2D; }
+int batch_rows_nocheck(int ndim, int dcase, int arith, int variant) {
+    if (ndim == 2) return variant == 1 ? 8 : 16;
+    return (arith == PTD_ARITH_FAST && dcase != PTD_STEP) ? 12 : 16;
+}
+
+template <int DC, int ARITH>
+static hipError_t batch_sweep_t(hipStream_t s, int ndim, int rows, bool check, BatchMember* mem,
+                                const BatchTile* tiles, int ntiles, int li) {
+    const int per = (ntiles + 7) / 8;
+    const dim3 g(8 * per);
+    if (ndim == 3) {
+        constexpr int RN = (ARITH == kFast && DC != PTD_STEP) ? 12 : 16;
+        if (check) {
+            if (rows != 16) return hipErrorInvalidValue;
+            hipLaunchKernelGGL((k_batch3d_t2<DC, ARITH, 16, true>), g, dim3(64, 16), 0, s, mem, tiles, ntiles, per, li);
+        } else {
+            if (rows != RN) return hipErrorInvalidValue;
+            hipLaunchKernelGGL((k_batch3d_t2<DC, ARITH, RN, false>), g, dim3(64, RN), 0, s, mem, tiles, ntiles, per, li);
+        }
+    } else {
+        if (check) {
+            if (rows != kRows2D) return hipErrorInvalidValue;
+            hipLaunchKernelGGL((k_batch2d<DC, ARITH, kRows2D, true>), g, dim3(64, kRows2D), 0, s, mem, tiles, ntiles, per, li);
+        } else if (rows == 8) {
+            hipLaunchKernelGGL((k_batch2d<DC, ARITH, 8, false>), g, dim3(64, 8), 0, s, mem, tiles, ntiles, per, li);
+        } else if (rows == 16) {
+            hipLaunchKernelGGL((k_batch2d<DC, ARITH, 16, false>), g, dim3(64, 16), 0, s, mem, tiles, ntiles, per, li);
+        } else {
+            return hipErrorInvalidValue;
+        }
+    }
+    return hipGetLastError();
+}
+
+hipError_t launch_batch_sweep(hipStream_t s, int ndim, int dcase, int arith, int rows, bool check,
+                              BatchMember* mem, const BatchTile* tiles, int ntiles, int li) {
+    if (ntiles <= 0) return hipSuccess;
+    const bool fast = arith == PTD_ARITH_FAST;
+    switch (dcase) {
+        case PTD_LIN:
+            return fast ? batch_sweep_t<PTD_LIN, kFast>(s, ndim, rows, check, mem, tiles, ntiles, li)
+                        : batch_sweep_t<PTD_LIN, kExact>(s, ndim, rows, check, mem, tiles, ntiles, li);
+        case PTD_STEP:
+            return fast ? batch_sweep_t<PTD_STEP, kFast>(s, ndim, rows, check, mem, tiles, ntiles, li)
+                        : batch_sweep_t<PTD_STEP, kExact>(s, ndim, rows, check, mem, tiles, ntiles, li);
+        default:
+            return fast ? batch_sweep_t<PTD_NONLIN, kFast>(s, ndim, rows, check, mem, tiles, ntiles, li)
+                        : batch_sweep_t<PTD_NONLIN, kExact>(s, ndim, rows, check, mem, tiles, ntiles, li);
+    }
+}
+
+// per member of a group: (sum, max) of its checking launch's tile partials, in the fixed order of the
+// single-problem reduction
+__global__ void __launch_bounds__(kBlock)
+k_batch_reduce(BatchMember* __restrict__ mem, const int32_t* __restrict__ members) {
+    BatchMember& M = mem[members[blockIdx.x]];
+    if (!M.active) return;
+    const double2 res = reduce_partials_block(M.partial, M.ntiles_check);
+    if (threadIdx.x == 0) M.red = res;
+}
+
+hipError_t launch_batch_reduce(hipStream_t s, BatchMember* mem, const int32_t* members, int n) {
+    if (n <= 0) return hipSuccess;
+    hipLaunchKernelGGL(k_batch_reduce, dim3(n), dim3(kBlock), 0, s, mem, members);
+    return hipGetLastError();
+}
+
+// The PT loop of every member after one check block of nout iterations
+// (scripts/diff_3D/diff_3D_nonlin_multixpu.jl:104-118): iter += nout; err from the check; the step
+// ends when !(err > tol) or iter >= itmax (then Hold .= H, next step); the member is done after nt
+// steps or on NaN. One thread per member; the number of members still running goes to *active_count.
+__global__ void __launch_bounds__(kBatchMaxMembers)
+k_batch_control(BatchMember* __restrict__ mem, int n, int64_t* __restrict__ iters, double* __restrict__ errs,
+                int32_t* __restrict__ active_count, int nt_cap) {
+    const int t = threadIdx.x;
+    int running = 0;
+    if (t < n) {
+        BatchMember& M = mem[t];
+        M.assign = 0;
+        if (M.active) {
+            M.iter += M.nout;
+            M.hpar = (M.hpar + M.lpb) & 1;
+            M.qpar = (M.qpar + M.lpb) & 1;
+            const double rms = sqrt(M.red.x) / sqrt(M.norm_len);
+            const double err = M.stop_norm == PTD_NORM_MAX ? M.red.y : rms;
+            if (!(err > M.tol) || M.iter >= M.itmax) {
+                if (M.step < nt_cap) {
+                    iters[(int64_t)t * nt_cap + M.step] = M.iter;
+                    errs[(int64_t)t * nt_cap + M.step] = err;
+                }
+                M.step += 1;
+                M.iter = 0;
+                if (isnan(err)) {
+                    M.status = PTD_ERR_NAN;
+                    M.active = 0;
+                    M.t_done = (uint64_t)wall_clock64();
+                } else {
+                    M.assign = 1;
+                    if (M.step >= M.nt) {
+                        M.active = 0;
+                        M.t_done = (uint64_t)wall_clock64();
+                    }
+                }
+            }
+        }
+        running = M.active;
+    }
+    const int cnt = __syncthreads_count(running);
+    if (t == 0) *active_count = cnt;
+}
+
+hipError_t launch_batch_control(hipStream_t s, BatchMember* mem, int n, int64_t* iters, double* errs,
+                                int32_t* active_count, int nt_cap) {
+    if (n <= 0 || n > kBatchMaxMembers) return hipErrorInvalidValue;
+    hipLaunchKernelGGL(k_batch_control, dim3(1), dim3(kBatchMaxMembers), 0, s, mem, n, iters, errs, active_count,
+                       nt_cap);
+    return hipGetLastError();
+}
+
+// Hold .= H for the members whose step just ended (kBatchAssignBlocks blocks per member)
+constexpr int kBatchAssignBlocks = 64;
+__global__ void __launch_bounds__(kBlock) k_batch_assign(BatchMember* __restrict__ mem) {
+    const BatchMember& M = mem[blockIdx.x / kBatchAssignBlocks];
+    if (!M.assign) return;
+    const double* __restrict__ H = M.H[M.hpar];
+    double* __restrict__ Ho = M.Hold;
+    const int64_t stride = (int64_t)kBatchAssignBlocks * kBlock;
+    for (int64_t i = (int64_t)(blockIdx.x % kBatchAssignBlocks) * kBlock + threadIdx.x; i < M.nH; i += stride)
+        Ho[i] = H[i];
+}
+
+hipError_t launch_batch_assign(hipStream_t s, BatchMember* mem, int n) {
+    if (n <= 0) return hipSuccess;
+    hipLaunchKernelGGL(k_batch_assign, dim3(n * kBatchAssignBlocks), dim3(kBlock), 0, s, mem);
+    return hipGetLastError();
+}
+
+__global__ void k_batch_clock(uint64_t* t) {
+    if (threadIdx.x == 0) *t = (uint64_t)wall_clock64();
+}
+
+hipError_t launch_batch_clock(hipStream_t s, uint64_t* t) {
+    hipLaunchKernelGGL(k_batch_clock, dim3(1), dim3(64), 0, s, t);
+    return hipGetLastError();
 }
 
 }  // namespace ptd

@@ -55,3 +55,30 @@ def test_null_context_is_an_argument_error(pkg):
     assert lib.ptd_iterate(None, 1) == abi.ERR_ARG
     assert lib.ptd_check_res(None, None, None) == abi.ERR_ARG
     assert b"NULL" in lib.ptd_last_error()
+
+
+@pytest.mark.parametrize("what", ["1d", "fp32", "perf", "odd_nout", "itmax", "nout_mismatch", "empty"])
+def test_batch_rejects_invalid_members_before_device(what, pkg):
+    from importlib import import_module
+    abi = import_module("ptd_amd._abi")
+    lin2 = pkg.make_params(pkg.SCRIPTS["diff_2D_lin"], 32, 32)
+    nl3 = pkg.make_params(pkg.SCRIPTS["diff_3D_nonlin_multixpu"], 16, 16, 16)
+    members = [lin2, nl3]
+    if what == "1d":
+        members[0] = pkg.make_params(pkg.SCRIPTS["diff_1D_lin"], 32)
+    elif what == "fp32":
+        members[1].precision = abi.FP32
+    elif what == "perf":
+        members[1].form = abi.PERF
+    elif what == "odd_nout":
+        for m in members:
+            m.nout = 5
+    elif what == "itmax":
+        members[0].itmax = 1005
+    elif what == "nout_mismatch":
+        members[1].nout = 20
+    elif what == "empty":
+        members = []
+    with pytest.raises(pkg.PTDError) as e:
+        pkg.Batch(members)
+    assert e.value.status == abi.ERR_ARG

@@ -1,0 +1,105 @@
+"""Batches of independent solves (ptd_batch_*) against the same problems solved one at a time
+(ptd_solve through a context): every member's H, Hold, fluxes, per-step iteration counts and errors
+must be identical, bit for bit, for 2D and 3D members of every D type, in EXACT and FAST arithmetic,
+over two consecutive solve calls (the batch replays one captured check-block graph many times; the
+single-problem solves run eagerly)."""
+import numpy as np
+import pytest
+
+pytestmark = pytest.mark.gpu
+
+# (script, shape, amplitude, centre fraction)
+MEMBERS = [
+    ("diff_2D_lin", (40, 33), 1.0, (0.4, 0.6)),
+    ("diff_2D_linstep", (64, 48), 1.5, (0.5, 0.5)),
+    ("diff_2D_nonlin", (37, 37), 0.8, (0.6, 0.4)),
+    ("diff_3D_lin_multixpu", (26, 20, 18), 1.2, (0.5, 0.4, 0.6)),
+    ("diff_3D_linstep_multixpu", (30, 24, 20), 2.0, (0.5, 0.5, 0.5)),
+    ("diff_3D_nonlin_multixpu", (28, 28, 28), 1.0, (0.45, 0.55, 0.5)),
+    ("diff_3D_nonlin_multixpu", (67, 35, 21), 0.7, (0.3, 0.7, 0.5)),
+]
+
+
+def _gauss(spec, shape, amp, centre):
+    nd = len(shape)
+    ls = [spec["lx"], spec.get("ly", 0.0), spec.get("lz", 0.0)][:nd]
+    r2 = 0.0
+    for d in range(nd):
+        dx = ls[d] / shape[d]
+        xc = dx / 2 + dx * np.arange(shape[d]) - centre[d] * ls[d]
+        sh = [1] * nd
+        sh[d] = shape[d]
+        r2 = r2 + (xc ** 2).reshape(sh)
+    return np.asfortranarray(amp * np.exp(-r2))
+
+
+def _setup(pkg, arith):
+    params, fields = [], []
+    for script, shape, amp, centre in MEMBERS:
+        spec = pkg.SCRIPTS[script]
+        p = pkg.make_params(spec, *shape, arith=arith)
+        H0 = _gauss(spec, shape, amp, centre)
+        D = pkg.step_field(spec, *shape) if spec["dcase"] == pkg.STEP else None
+        params.append(p)
+        fields.append((H0, D))
+    return params, fields
+
+
+@pytest.mark.parametrize("arith", [0, 1], ids=["exact", "fast"])
+def test_batch_equals_single_solves(arith, pkg):
+    print(f"[gpu] batch vs single solves, arith={arith}", flush=True)
+    params, fields = _setup(pkg, arith)
+    nt = 5
+    with pkg.Batch(params) as b:
+        for m, (H0, D) in enumerate(fields):
+            b.set(m, "H", H0)
+            b.set(m, "Hold", H0)
+            if D is not None:
+                b.set(m, "D", D)
+        runs = []
+        for call in range(2):
+            it, er, lat, st, total = b.solve(nt)
+            assert np.all(st == 0)
+            assert np.all(lat > 0) and np.all(lat <= total * 1.01 + 1.0)
+            runs.append((it, er, [b.get(m, "H") for m in range(len(params))],
+                         [b.get(m, "Hold") for m in range(len(params))],
+                         [[b.get(m, f) for f in ("qHx", "qHy", "qHz")[:params[m].ndim]] for m in range(len(params))]))
+        tz, rows, ms = b.tiling()
+        assert tz in (8, 16, 32) and rows in (8, 16)
+    for m, (H0, D) in enumerate(fields):
+        with pkg.Context(params[m]) as ctx:
+            ctx.set("H", H0)
+            ctx.set("Hold", H0)
+            if D is not None:
+                ctx.set("D", D)
+            for call in range(2):
+                it, er = ctx.solve(nt)
+                bit, ber, bH, bHo, bq = runs[call]
+                assert np.array_equal(bit[m], it), f"member {m} call {call}: {bit[m]} vs {it}"
+                assert np.array_equal(ber[m], er), f"member {m} call {call}"
+                assert np.array_equal(bH[m], ctx.get("H")), f"member {m} call {call}: H"
+                assert np.array_equal(bHo[m], ctx.get("Hold")), f"member {m} call {call}: Hold"
+                for a, f in enumerate(("qHx", "qHy", "qHz")[:params[m].ndim]):
+                    assert np.array_equal(bq[m][a], ctx.get(f)), f"member {m} call {call}: {f}"
+
+
+def test_batch_workload_sample(pkg):
+    """A reduced draw of the batched workload (small sizes): every member converges, counts are
+    multiples of nout, latencies are ordered within the call."""
+    from importlib import import_module
+    B = import_module("ptd_amd.batch")
+    w = B.workload(count=12, seed=3, sizes_2d=[128, 192], sizes_3d=[64, 96])
+    params = [B.member_params(m) for m in w]
+    print(f"[gpu] batch workload sample: {len(w)} members", flush=True)
+    with pkg.Batch(params) as b:
+        for i, m in enumerate(w):
+            H0, D = B.member_fields(m)
+            b.set(i, "H", H0)
+            b.set(i, "Hold", H0)
+            if D is not None:
+                b.set(i, "D", D)
+        it, er, lat, st, total = b.solve(5)
+    assert np.all(st == 0)
+    assert np.all(it > 0) and np.all(it % 10 == 0)
+    assert np.all(er <= 1e-8)
+    assert 0 < B.percentile(lat, 50) <= B.percentile(lat, 99) <= total * 1.01 + 1.0
