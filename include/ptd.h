@@ -177,6 +177,20 @@ int32_t ptd_pt_solve(ptd_ctx* ctx, int64_t* iters, double* err_rms, double* err_
  * iters[nt] and errs[nt] (may be NULL) receive per-step iteration counts and final errors. */
 int32_t ptd_solve(ptd_ctx* ctx, int32_t nt, int64_t* iters, double* errs);
 
+/* ---- gradients (lin / step cases) ----------------------------------------------------------------
+ *
+ * No reference counterpart: the reference has no derivative. Runs ptd_solve(nt) from the context's
+ * current state while keeping H^0..H^nt on the device, then the adjoint of the nt backward-Euler steps
+ * (for D independent of H the step operator A = I/dt - div(av(D) grad .) is symmetric, so each adjoint
+ * solve A mu = r is the same PT solve, with homogeneous boundary values), and returns
+ *   *loss    = L = 1/2 sum over the inner cells of (H^nt - H_obs)^2,
+ *   grad_d   = dL/dD per cell (nx*ny*nz; in the lin case the sensitivity to a per-cell D at d0),
+ *   grad_h0  = dL/dH0 per cell (inner cells: mu_1/dt; boundary cells: their Dirichlet coupling),
+ *   iters_fwd[nt], iters_adj[nt] = PT iterations of each forward / adjoint solve.
+ * Any output may be NULL. The context ends as after ptd_solve(nt). float64, PTD_LIN or PTD_STEP. */
+int32_t ptd_grad_solve(ptd_ctx* ctx, int32_t nt, const double* h_obs, int64_t count, double* loss,
+                       double* grad_d, double* grad_h0, int64_t* iters_fwd, int64_t* iters_adj);
+
 /* ---- batches of independent solves ------------------------------------------------------------
  *
  * No reference counterpart: the reference solves one problem per script run. A batch holds n

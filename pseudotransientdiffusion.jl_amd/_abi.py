@@ -32,6 +32,7 @@ EXPORTS = (
     "ptd_compute_iter_params", "ptd_compute_flux", "ptd_compute_update", "ptd_compute_flux_res",
     "ptd_check_res", "ptd_assign",
     "ptd_iterate", "ptd_iterate_timed", "ptd_pt_solve", "ptd_solve",
+    "ptd_grad_solve",
     "ptd_batch_create", "ptd_batch_destroy", "ptd_batch_size", "ptd_batch_device_bytes",
     "ptd_batch_set_field", "ptd_batch_get_field", "ptd_batch_solve", "ptd_batch_tiling",
 )
@@ -97,6 +98,7 @@ def load():
         "ptd_iterate_timed": (i32, [vp, i64, dp]),
         "ptd_pt_solve": (i32, [vp, i64p, dp, dp]),
         "ptd_solve": (i32, [vp, i32, i64p, dp]),
+        "ptd_grad_solve": (i32, [vp, i32, dp, i64, dp, dp, dp, i64p, i64p]),
         "ptd_batch_create": (i32, [pp, i32, i32, ctypes.POINTER(vp)]),
         "ptd_batch_destroy": (i32, [vp]),
         "ptd_batch_size": (i32, [vp]),

@@ -417,7 +417,7 @@ int32_t ptd_ctx_create(const ptd_params* params, int32_t device, ptd_ctx** out) 
         if (c->p.dcase == PTD_STEP) PTD_ALLOC(&c->Df, c->nH, sizeof(float));
         if (c->p.dcase == PTD_NONLIN) PTD_ALLOC(&c->hcf, c->nH, sizeof(float));
     }
-    c->npartial = ptd::partial_capacity(c->p, c->nR);
+    c->npartial = ptd::partial_capacity(c->p, c->nH);  // per-cell reductions (adjoint source) cover nH
     PTD_ALLOC((void**)&c->partial, 2 * c->npartial, sizeof(double));
     PTD_ALLOC((void**)&c->dred, 2, sizeof(double));
 #undef PTD_ALLOC

@@ -1,0 +1,122 @@
+// ptd_grad.hip -- gradient of a loss on the final iterate of the lin / step solves (include/ptd.h:
+// ptd_grad_solve): L = 1/2 sum over inner cells (H(T) - H_obs)^2 with respect to the per-cell D (step
+// case; the scalar d0 spread over the cells in the lin case) and to the initial condition H0.
+//
+// For D independent of H every physical step solves A(D) H^n = H^{n-1}/dt with
+// A = I/dt - div(av(D) grad .) on the inner cells (scripts/diff_3D/diff_3D_linstep_multixpu.jl:29-47),
+// and A is symmetric. The adjoint of the nt steps is therefore
+//     A mu_nt = H^nt - H_obs,     A mu_n = mu_{n+1}/dt   (n = nt-1 .. 1),
+// i.e. the same PT loop run on mu with homogeneous Dirichlet values: Hold = (H^nt - H_obs)*dt for the
+// first solve, then Hold .= H between solves. The forward iterates H^0..H^nt are kept on the device
+// (checkpoints); after each adjoint solve one kernel gathers the step's face sensitivities
+// dL/dD_f = (H_b - H_a)(mu_a - mu_b)/h^2 into the cells (half to each side), and the boundary coupling
+// into dL/dH0; dL/dH0 = mu_1/dt on the inner cells. The context ends as after ptd_solve(nt).
+#include <hip/hip_runtime.h>
+
+#include <cmath>
+#include <string>
+#include <vector>
+
+#include "ptd_internal.hpp"
+
+extern "C" int32_t ptd_pt_solve(ptd_ctx* ctx, int64_t* iters, double* err_rms, double* err_max);
+extern "C" int32_t ptd_assign(ptd_ctx* ctx);
+
+namespace {
+
+#define PTD_HIPG(expr)                                                                             \
+    do {                                                                                           \
+        hipError_t e_ = (expr);                                                                    \
+        if (e_ != hipSuccess) {                                                                    \
+            status = ptd::set_error(PTD_ERR_HIP, std::string(#expr) + ": " + hipGetErrorString(e_)); \
+            goto done;                                                                             \
+        }                                                                                          \
+    } while (0)
+
+}  // namespace
+
+extern "C" int32_t ptd_grad_solve(ptd_ctx* c, int32_t nt, const double* h_obs, int64_t count, double* loss,
+                                  double* grad_d, double* grad_h0, int64_t* iters_fwd, int64_t* iters_adj) {
+    if (!c) return ptd::set_error(PTD_ERR_ARG, "ctx is NULL");
+    if (nt < 1 || !h_obs) return ptd::set_error(PTD_ERR_ARG, "need nt >= 1 and H_obs");
+    if (count != c->nH) return ptd::set_error(PTD_ERR_ARG, "H_obs must have nx*ny*nz values");
+    if (c->p.dcase == PTD_NONLIN) return ptd::set_error(PTD_ERR_STATE, "gradients are for D independent of H (lin / step)");
+    if (c->p.precision != PTD_FP64) return ptd::set_error(PTD_ERR_STATE, "gradients are float64");
+    if (hipSetDevice(c->device) != hipSuccess) return ptd::set_error(PTD_ERR_HIP, "hipSetDevice failed");
+
+    int status = PTD_OK;
+    const size_t nH = (size_t)c->nH;
+    double *ckpt = nullptr, *hobs = nullptr, *gD = nullptr, *gH0 = nullptr;
+    double* qbak[3] = {nullptr, nullptr, nullptr};
+    double red[2] = {0, 0};
+    {
+        PTD_HIPG(hipMalloc((void**)&ckpt, (size_t)(nt + 1) * nH * 8));
+        PTD_HIPG(hipMalloc((void**)&hobs, nH * 8));
+        PTD_HIPG(hipMalloc((void**)&gD, nH * 8));
+        PTD_HIPG(hipMalloc((void**)&gH0, nH * 8));
+        for (int a = 0; a < 3; ++a)
+            if (c->nQ[a] > 0) PTD_HIPG(hipMalloc((void**)&qbak[a], (size_t)c->nQ[a] * 8));
+        PTD_HIPG(hipMemsetAsync(gD, 0, nH * 8, c->stream));
+        PTD_HIPG(hipMemsetAsync(gH0, 0, nH * 8, c->stream));
+        PTD_HIPG(hipMemcpyAsync(hobs, h_obs, nH * 8, hipMemcpyHostToDevice, c->stream));
+
+        // forward: ptd_solve(nt), keeping H^0..H^nt
+        PTD_HIPG(hipMemcpyAsync(ckpt, c->H[c->hcur], nH * 8, hipMemcpyDeviceToDevice, c->stream));
+        for (int n = 1; n <= nt; ++n) {
+            int64_t it = 0;
+            double rms = 0, mx = 0;
+            status = ptd_pt_solve(c, &it, &rms, &mx);
+            if (iters_fwd) iters_fwd[n - 1] = it;
+            if (status != PTD_OK) goto done;
+            PTD_HIPG(hipMemcpyAsync(ckpt + (size_t)n * nH, c->H[c->hcur], nH * 8, hipMemcpyDeviceToDevice, c->stream));
+            status = ptd_assign(c);
+            if (status != PTD_OK) goto done;
+        }
+        for (int a = 0; a < 3; ++a)
+            if (qbak[a])
+                PTD_HIPG(hipMemcpyAsync(qbak[a], c->q[a][c->qcur], (size_t)c->nQ[a] * 8, hipMemcpyDeviceToDevice, c->stream));
+
+        // loss and the adjoint source: Hold = H = (H^nt - H_obs)*dt (0 on the boundary), fluxes from 0
+        PTD_HIPG(ptd::launch_adj_source(c, (const double*)c->H[c->hcur], hobs, (double*)c->Hold));
+        PTD_HIPG(hipMemcpyAsync(red, c->dred, 2 * sizeof(double), hipMemcpyDeviceToHost, c->stream));
+        for (int b = 0; b < 2; ++b) PTD_HIPG(hipMemcpyAsync(c->H[b], c->Hold, nH * 8, hipMemcpyDeviceToDevice, c->stream));
+        for (int a = 0; a < 3; ++a)
+            for (int b = 0; b < 2; ++b)
+                if (c->q[a][b]) PTD_HIPG(hipMemsetAsync(c->q[a][b], 0, (size_t)c->nQ[a] * 8, c->stream));
+        PTD_HIPG(hipStreamSynchronize(c->stream));
+        if (loss) *loss = red[0];
+
+        // adjoint: mu_nt, ..., mu_1, each followed by its gradient contribution
+        for (int n = nt; n >= 1; --n) {
+            int64_t it = 0;
+            double rms = 0, mx = 0;
+            status = ptd_pt_solve(c, &it, &rms, &mx);
+            if (iters_adj) iters_adj[n - 1] = it;
+            if (status != PTD_OK) goto done;
+            PTD_HIPG(ptd::launch_grad_accum(c, (const double*)c->H[c->hcur], ckpt + (size_t)n * nH, gD, gH0, n == 1));
+            if (n > 1) {
+                status = ptd_assign(c);  // Hold = mu_n: the next solve is A mu_{n-1} = mu_n/dt
+                if (status != PTD_OK) goto done;
+            }
+        }
+        if (grad_d) PTD_HIPG(hipMemcpyAsync(grad_d, gD, nH * 8, hipMemcpyDeviceToHost, c->stream));
+        if (grad_h0) PTD_HIPG(hipMemcpyAsync(grad_h0, gH0, nH * 8, hipMemcpyDeviceToHost, c->stream));
+
+        // the context as after ptd_solve(nt): H = Hold = H^nt, the forward fluxes
+        for (int b = 0; b < 2; ++b)
+            PTD_HIPG(hipMemcpyAsync(c->H[b], ckpt + (size_t)nt * nH, nH * 8, hipMemcpyDeviceToDevice, c->stream));
+        PTD_HIPG(hipMemcpyAsync(c->Hold, ckpt + (size_t)nt * nH, nH * 8, hipMemcpyDeviceToDevice, c->stream));
+        for (int a = 0; a < 3; ++a)
+            if (qbak[a])
+                PTD_HIPG(hipMemcpyAsync(c->q[a][c->qcur], qbak[a], (size_t)c->nQ[a] * 8, hipMemcpyDeviceToDevice, c->stream));
+        PTD_HIPG(hipStreamSynchronize(c->stream));
+    }
+done:
+    (void)hipStreamSynchronize(c->stream);
+    (void)hipFree(ckpt);
+    (void)hipFree(hobs);
+    (void)hipFree(gD);
+    (void)hipFree(gH0);
+    for (int a = 0; a < 3; ++a) (void)hipFree(qbak[a]);
+    return status;
+}

@@ -431,6 +431,11 @@ hipError_t launch_f64_to_f32(ptd_ctx* c, float* dst, const double* src, int64_t 
 // is the equilibrium flux magnitude coef*dH/h of the float64 iterate H on each face
 hipError_t launch_flux_state(ptd_ctx* c, bool to_p);
 
+// gradients of the lin / step solves (ptd_grad.hip): adjoint source (H - Hobs)*dt with L's partials
+// reduced into c->dred, and one step's gradient contribution
+hipError_t launch_adj_source(ptd_ctx* c, const double* H, const double* Hobs, double* out);
+hipError_t launch_grad_accum(ptd_ctx* c, const double* mu, const double* Hn, double* gD, double* gH0, bool first);
+
 // ---- batches of independent float64 solves (ptd_batch.hip) -----------------------------------
 
 // Device-resident state of one member of a batch. The sweep kernels read the member's fields and

@@ -1203,6 +1203,99 @@ int64_t partial_capacity(const ptd_params& p, int64_t nR) {
 }
 
 // ---------------------------------------------------------------------------------------------
+// gradients of the lin / step solves (adjoint)
+// ---------------------------------------------------------------------------------------------
+
+// Adjoint source of L = 1/2 sum_inner (H - Hobs)^2: out = (H - Hobs)*dt on the inner cells, 0 on the
+// boundary cells (the adjoint has homogeneous Dirichlet values); per-block partials of r^2/2 for L.
+__global__ void __launch_bounds__(kBlock)
+k_adj_source(KP<double> p, const double* __restrict__ H, const double* __restrict__ Hobs, double* __restrict__ out,
+             double2* __restrict__ partial, int64_t n) {
+    const int64_t c = (int64_t)blockIdx.x * kBlock + threadIdx.x;
+    double l = 0.0;
+    if (c < n) {
+        const int64_t i = c % p.nx, rest = c / p.nx, j = rest % p.ny, k = rest / p.ny;
+        const bool in = i >= 1 && i <= p.nx - 2 && (p.hy == 0 || (j >= 1 && j <= p.ny - 2)) &&
+                        (p.hz == 0 || (k >= 1 && k <= p.nz - 2));
+        const double r = in ? H[c] - Hobs[c] : 0.0;
+        out[c] = r * p.dt;
+        l = 0.5 * r * r;
+    }
+    block_sum_max(l, 0.0, threadIdx.x, kBlock, &partial[blockIdx.x]);
+}
+
+// Accumulates one step's contribution to the gradient, per cell (gather over the faces at the cell):
+//   gD[c]  += 1/2 sum_{faces f at c} (H_b - H_a)(mu_a - mu_b)/h^2        (face f between a < b)
+//   gH0[c] += sum_{faces f between boundary c and inner i} mu_i D_f/h^2  (c a boundary cell)
+// plus gH0 += mu/dt on the inner cells for the first step. A face exists where the reference's flux
+// array has it: transverse indices inner. mu is 0 on the boundary cells.
+template <int NDIM, bool STEP>
+__global__ void __launch_bounds__(kBlock)
+k_grad_accum(KP<double> p, const double* __restrict__ mu, const double* __restrict__ Hn, const double* __restrict__ D,
+             double* __restrict__ gD, double* __restrict__ gH0, int first, int64_t n) {
+    const int64_t c = (int64_t)blockIdx.x * kBlock + threadIdx.x;
+    if (c >= n) return;
+    const int64_t nx = p.nx, ny = p.ny, nz = p.nz;
+    const int64_t i = c % nx, rest = c / nx, j = rest % ny, k = rest / ny;
+    const int64_t idx[3] = {i, j, k};
+    const int64_t nn[3] = {nx, ny, nz};
+    const int64_t st[3] = {1, nx, nx * ny};
+    const double w[3] = {p._dx * p._dx, p._dy * p._dy, p._dz * p._dz};
+    auto inner = [&](int ax) { return idx[ax] >= 1 && idx[ax] <= nn[ax] - 2; };
+    const bool cin = inner(0) && (NDIM < 2 || inner(1)) && (NDIM < 3 || inner(2));
+    double g = 0.0, gb = 0.0;
+#pragma unroll
+    for (int ax = 0; ax < NDIM; ++ax) {
+        bool trans = true;  // transverse indices inner: the flux array has faces on this line
+#pragma unroll
+        for (int o = 0; o < NDIM; ++o)
+            if (o != ax) trans = trans && inner(o);
+        if (!trans) continue;
+#pragma unroll
+        for (int side = 0; side < 2; ++side) {  // 0: face (c-1 | c), 1: face (c | c+1)
+            const bool has = side == 0 ? idx[ax] >= 1 : idx[ax] <= nn[ax] - 2;
+            if (!has) continue;
+            const int64_t a = side == 0 ? c - st[ax] : c, b = side == 0 ? c : c + st[ax];
+            g += 0.5 * (Hn[b] - Hn[a]) * (mu[a] - mu[b]) * w[ax];
+            if (!cin) {
+                const int64_t o = side == 0 ? a : b;  // the neighbour across the face
+                const double Df = STEP ? (D[a] + D[b]) * 0.5 : p.d0;
+                gb += mu[o] * Df * w[ax];
+            }
+        }
+    }
+    gD[c] += g;
+    if (cin) {
+        if (first) gH0[c] += mu[c] * p._dt;
+    } else {
+        gH0[c] += gb;
+    }
+}
+
+hipError_t launch_adj_source(ptd_ctx* c, const double* H, const double* Hobs, double* out) {
+    const KP<double> kp = make_kp<double>(c->p);
+    const unsigned g = grid_for(c->nH);
+    if ((int64_t)g > c->npartial) return hipErrorInvalidValue;
+    hipLaunchKernelGGL(k_adj_source, dim3(g), dim3(kBlock), 0, c->stream, kp, H, Hobs, out, (double2*)c->partial, c->nH);
+    hipLaunchKernelGGL(k_reduce_partials, dim3(1), dim3(kBlock), 0, c->stream, (const double2*)c->partial, (int64_t)g,
+                       c->dred);
+    return hipGetLastError();
+}
+
+hipError_t launch_grad_accum(ptd_ctx* c, const double* mu, const double* Hn, double* gD, double* gH0, bool first) {
+    const KP<double> kp = make_kp<double>(c->p);
+    const dim3 g(grid_for(c->nH)), bl(kBlock);
+    const double* D = (const double*)c->D;
+    const bool step = c->p.dcase == PTD_STEP;
+#define PTD_GRAD(ND)                                                                                          \
+    if (step) hipLaunchKernelGGL((k_grad_accum<ND, true>), g, bl, 0, c->stream, kp, mu, Hn, D, gD, gH0, (int)first, c->nH); \
+    else hipLaunchKernelGGL((k_grad_accum<ND, false>), g, bl, 0, c->stream, kp, mu, Hn, D, gD, gH0, (int)first, c->nH);
+    if (c->p.ndim == 1) { PTD_GRAD(1) } else if (c->p.ndim == 2) { PTD_GRAD(2) } else { PTD_GRAD(3) }
+#undef PTD_GRAD
+    return hipGetLastError();
+}
+
+// ---------------------------------------------------------------------------------------------
 // batches of independent float64 solves
 // ---------------------------------------------------------------------------------------------
 //

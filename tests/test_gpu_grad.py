@@ -1,0 +1,101 @@
+"""Gradients of the lin / step solves through the C ABI (ptd_grad_solve) against the oracle's exact
+adjoint and forward mode (oracle/be_linear.py, pinned by its own finite-difference tests), and against
+central finite differences of GPU solves."""
+import numpy as np
+import pytest
+
+from oracle.be_linear import BELinear
+
+pytestmark = pytest.mark.gpu
+
+
+def _setup(pkg, script, shape, tol, seed=0, arith=1):
+    spec = pkg.SCRIPTS[script]
+    p = pkg.make_params(spec, *shape, arith=arith)
+    p.tol = tol
+    nd = spec["ndim"]
+    rng = np.random.default_rng(seed)
+    ls = [spec["lx"], spec.get("ly", 0.0), spec.get("lz", 0.0)][:nd]
+    r2 = 0.0
+    for d in range(nd):
+        dx = ls[d] / shape[d]
+        xc = dx / 2 + dx * np.arange(shape[d]) - 0.5 * ls[d]
+        sh = [1] * nd
+        sh[d] = shape[d]
+        r2 = r2 + (xc ** 2).reshape(sh)
+    H0 = np.asfortranarray(np.exp(-r2 / 4.0))
+    D = pkg.step_field(spec, *shape) if spec["dcase"] == pkg.STEP else np.full(shape, p.d0, order="F")
+    Hobs = np.asfortranarray(H0 * (1.0 + 0.05 * rng.standard_normal(shape)))
+    h = [p.dx, p.dy, p.dz][:nd]
+    return p, H0, D, Hobs, h
+
+
+def _grad_gpu(pkg, p, H0, D, Hobs, nt):
+    with pkg.Context(p) as ctx:
+        ctx.set("H", H0)
+        ctx.set("Hold", H0)
+        if p.dcase == pkg.STEP:
+            ctx.set("D", D)
+        return ctx.grad_solve(nt, Hobs)
+
+
+@pytest.mark.parametrize("script,shape", [("diff_2D_linstep", (40, 36)), ("diff_3D_linstep_multixpu", (32, 32, 32)),
+                                          ("diff_2D_lin", (30, 30))])
+def test_gradient_matches_oracle_adjoint_and_forward_mode(script, shape, pkg):
+    print(f"[gpu] gradient vs oracle {script} {shape}", flush=True)
+    nt = 4
+    p, H0, D, Hobs, h = _setup(pkg, script, shape, tol=1e-12)
+    g = _grad_gpu(pkg, p, H0, D, Hobs, nt)
+    be = BELinear(D, h, p.dt)
+    L, gD, gH0 = be.gradient(H0, Hobs, nt)
+    assert abs(g["loss"] - L) <= 1e-9 * L
+    assert np.max(np.abs(g["grad_D"] - gD)) <= 1e-6 * np.max(np.abs(gD))
+    assert np.max(np.abs(g["grad_H0"] - gH0)) <= 1e-6 * np.max(np.abs(gH0))
+    rng = np.random.default_rng(7)
+    for _ in range(20):  # forward mode of the reference system along random directions
+        vD = rng.standard_normal(shape)
+        vH = rng.standard_normal(shape)
+        jv = be.jvp(H0, Hobs, nt, vD, vH)
+        gv = float(np.sum(g["grad_D"] * vD) + np.sum(g["grad_H0"] * vH))
+        assert abs(gv - jv) <= 1e-6 * max(abs(jv), 1e-30), (gv, jv)
+
+
+def test_gradient_matches_finite_differences_on_gpu(pkg):
+    print("[gpu] gradient vs finite differences 48^3", flush=True)
+    script, shape, nt = "diff_3D_linstep_multixpu", (48, 48, 48), 3
+    p, H0, D, Hobs, h = _setup(pkg, script, shape, tol=1e-12, seed=1)
+    g = _grad_gpu(pkg, p, H0, D, Hobs, nt)
+    inner = (slice(1, -1),) * 3
+
+    def loss(Dv, H0v):
+        with pkg.Context(p) as ctx:
+            ctx.set("H", H0v)
+            ctx.set("Hold", H0v)
+            ctx.set("D", Dv)
+            ctx.solve(nt)
+            r = ctx.get("H")[inner] - Hobs[inner]
+            return 0.5 * float(np.sum(r * r))
+
+    rng = np.random.default_rng(11)
+    for _ in range(4):
+        vD = np.asfortranarray(rng.standard_normal(shape) * D)   # relative perturbations of D
+        vH = np.asfortranarray(rng.standard_normal(shape) * 0.1)
+        eps = 1e-5
+        fd = (loss(D + eps * vD, H0 + eps * vH) - loss(D - eps * vD, H0 - eps * vH)) / (2 * eps)
+        gv = float(np.sum(g["grad_D"] * vD) + np.sum(g["grad_H0"] * vH))
+        assert abs(fd - gv) <= 1e-5 * abs(gv), (fd, gv)
+
+
+def test_context_ends_as_after_solve(pkg):
+    script, shape, nt = "diff_3D_linstep_multixpu", (30, 26, 22), 3
+    p, H0, D, Hobs, h = _setup(pkg, script, shape, tol=1e-8)
+    states = []
+    for use_grad in (True, False):
+        with pkg.Context(p) as ctx:
+            ctx.set("H", H0)
+            ctx.set("Hold", H0)
+            ctx.set("D", D)
+            its = ctx.grad_solve(nt, Hobs)["iters_fwd"] if use_grad else ctx.solve(nt)[0]
+            states.append((its, ctx.get("H"), ctx.get("Hold"), ctx.get("qHx"), ctx.get("qHz"), ctx.solve(1)[0]))
+    for a, b in zip(*states):
+        assert np.array_equal(a, b)
