@@ -103,6 +103,66 @@ def batch_leg(pkg, count, seed=0):
     return out
 
 
+def grad_leg(pkg, n, nfd, seed=0, nt=10):
+    """dL/dD of L = 1/2 sum_inner (H(T) - H_obs)^2 for the 3D step case: H_obs from a perturbed D
+    (log D_true = log D + s, s ~ N(0, 0.3^2) per cell smoothed by a Gaussian of 2 cells and rescaled to
+    standard deviation 0.3), gradient at D by ptd_grad_solve, checked against central differences of
+    GPU solves along `nfd` random directions. PT tolerance 1e-10 (tighter than the scripts' 1e-8, so
+    that solver noise stays far below the finite-difference signal)."""
+    import numpy as np
+    import scipy.ndimage
+
+    spec = pkg.SCRIPTS["diff_3D_linstep_multixpu"]
+    p = pkg.make_params(spec, n, n, n, arith=1)
+    p.tol = 1e-10
+    H0 = pkg.initial_condition(spec, n, n, n)
+    D = pkg.step_field(spec, n, n, n)
+    rng = np.random.Generator(np.random.PCG64(seed))
+    s = scipy.ndimage.gaussian_filter(rng.standard_normal(D.shape), sigma=2.0, mode="nearest")
+    s *= 0.3 / s.std()
+    D_true = np.asfortranarray(D * np.exp(s))
+    inner = (slice(1, -1),) * 3
+
+    def run(Dv, grad_obs=None):
+        with pkg.Context(p) as ctx:
+            ctx.set("H", H0)
+            ctx.set("Hold", H0)
+            ctx.set("D", Dv)
+            t0 = time.perf_counter()
+            if grad_obs is None:
+                its, _ = ctx.solve(nt)
+                dt = time.perf_counter() - t0
+                return ctx.get("H"), its, dt
+            g = ctx.grad_solve(nt, grad_obs)
+            return g, time.perf_counter() - t0
+
+    log(f"grad: {n}^3 step case, {nt} steps: H_obs from the perturbed D")
+    Hobs, its_obs, _ = run(D_true)
+    Hfwd, its_fwd, t_fwd = run(D)
+    log(f"grad: forward {t_fwd:.3f} s ({int(its_fwd.sum())} PT iterations); gradient")
+    g, t_grad = run(D, Hobs)
+    r = Hfwd[inner] - Hobs[inner]
+    loss_fwd = 0.5 * float(np.sum(r * r))
+    errs = []
+    for k in range(nfd):
+        v = np.asfortranarray(rng.standard_normal(D.shape) * D)
+        eps = 1e-3
+        Hp, _, _ = run(D + eps * v)
+        Hm, _, _ = run(D - eps * v)
+        rp, rm = Hp[inner] - Hobs[inner], Hm[inner] - Hobs[inner]
+        fd = (0.5 * float(np.sum(rp * rp)) - 0.5 * float(np.sum(rm * rm))) / (2 * eps)
+        gv = float(np.sum(g["grad_D"] * v))
+        errs.append(abs(fd - gv) / abs(gv))
+        log(f"grad: direction {k + 1}/{nfd}: adjoint {gv:.6e} finite differences {fd:.6e} rel {errs[-1]:.2e}")
+    return {"grid": n, "steps": nt, "case": "3D step (D2 = 1e-4 for x or y < L/2.2), log D_true ~ smoothed N(0, 0.3^2)",
+            "loss": g["loss"], "loss_from_forward": loss_fwd, "forward_s": round(t_fwd, 4),
+            "grad_solve_s": round(t_grad, 4), "adjoint_s": round(t_grad - t_fwd, 4),
+            "pt_iterations_forward": int(g["iters_fwd"].sum()), "pt_iterations_adjoint": int(g["iters_adj"].sum()),
+            "fd_directions": nfd, "fd_eps_relative": 1e-3,
+            "fd_rel_err_max": max(errs) if errs else None,
+            "fd_rel_err_median": float(np.median(errs)) if errs else None, "tol": p.tol}
+
+
 def cpu_baseline(pkg, n, seconds):
     """The oracle's PT iterations (OpenMP C restatement of the reference Threads path) at n^3 on this
     host: run iterations until `seconds` have elapsed (at least one), report T_eff."""
@@ -152,6 +212,10 @@ def main():
     ap.add_argument("--batch", type=int, default=0,
                     help="also solve the batched workload with this many members (96: 2D {128..1024}^2 and "
                          "3D {64..256}^3 grids, lin / step / H^3, 5 physical steps each)")
+    ap.add_argument("--grad", type=int, default=0,
+                    help="also run the gradient workload at this grid size (255: 3D step case, 10 physical "
+                         "steps, dL/dD by the adjoint, checked by central differences)")
+    ap.add_argument("--grad-fd", type=int, default=20, help="finite-difference directions of the gradient check")
     ap.add_argument("--calib-copy", action="store_true",
                     help="only run the 1 GiB copy used to scale rocprofv3 FETCH_SIZE/WRITE_SIZE, then exit")
     args = ap.parse_args()
@@ -294,6 +358,9 @@ def main():
 
     if args.batch:
         out["batch"] = batch_leg(pkg, args.batch)
+
+    if args.grad:
+        out["gradient"] = grad_leg(pkg, args.grad, args.grad_fd)
 
     if not args.no_cpu:
         log(f"CPU baseline (~{args.cpu_seconds:.0f} s)")
