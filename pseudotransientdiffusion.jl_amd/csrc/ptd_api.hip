@@ -265,23 +265,23 @@ int pt_solve_mixed(ptd_ctx* c, int64_t* iters, double* err_rms, double* err_max)
     constexpr double kInner = 1e-2;
     const ptd_params& p = c->p;
     if (p.dcase == PTD_STEP) PTD_HIP(ptd::launch_f64_to_f32(c, (float*)c->Df, (const double*)c->D, c->nH));
-    auto residual64 = [&](double* rms, double* mx) -> int {
-        PTD_HIP(ptd::launch_flux(c, 2));
-        PTD_HIP(ptd::launch_residual(c, true));
+    // residual of the float64 system at Hc, with the next correction's source, float32(Hc), delta = 0
+    auto outer_residual = [&](double* rms, double* mx) -> int {
+        PTD_HIP(ptd::launch_outer_residual(c));
         return fetch_errors(c, rms, mx);
     };
     double rms = 0, mx = 0;
-    int st = residual64(&rms, &mx);
+    int st = outer_residual(&rms, &mx);
     if (st != PTD_OK) return st;
     double err = p.stop_norm == PTD_NORM_MAX ? mx : rms;
     int64_t total = 0, outer = 0;
+    bool in_p = false;  // the correction fluxes p hold the flux state (q is stale)
     while (err > p.tol && total < p.itmax && !std::isnan(err)) {
-        PTD_HIP(ptd::launch_make_source(c));
-        if (p.dcase == PTD_NONLIN)
-            PTD_HIP(ptd::launch_f64_to_f32(c, (float*)c->hcf, (const double*)c->H[c->hcur], c->nH));
-        PTD_HIP(hipMemsetAsync(c->dl[c->dcur], 0, (size_t)c->nH * sizeof(float), c->stream));
-        // start from the true fluxes (as the reference keeps qHx.. across steps): p = q + g(Hc)
-        PTD_HIP(ptd::launch_flux_state(c, true));
+        if (!in_p) {
+            // start from the true fluxes (as the reference keeps qHx.. across steps): p = q + g(Hc)
+            PTD_HIP(ptd::launch_flux_state(c, true));
+            in_p = true;
+        }
         const double target = std::max(0.5 * p.tol, kInner * err);
         double best = HUGE_VAL;
         int stall = 0;
@@ -321,14 +321,15 @@ int pt_solve_mixed(ptd_ctx* c, int64_t* iters, double* err_rms, double* err_max)
         }
         total += it;
         if (met) c->inner_hint = it;
-        PTD_HIP(ptd::launch_flux_state(c, false));  // q = p - g(Hc), before Hc moves
-        PTD_HIP(ptd::launch_add_correction(c));
-        st = residual64(&rms, &mx);
+        // p moves to Hc + delta and Hc += delta (one pass), then the next correction's residual
+        PTD_HIP(ptd::launch_outer_update(c));
+        st = outer_residual(&rms, &mx);
         if (st != PTD_OK) return st;
         err = p.stop_norm == PTD_NORM_MAX ? mx : rms;
         ++outer;
         if (it == 0) break;
     }
+    if (in_p) PTD_HIP(ptd::launch_flux_state(c, false));  // q = p - g(Hc): the flux state in float64
     PTD_HIP(hipStreamSynchronize(c->stream));
     c->outer_last = outer;
     if (iters) *iters = total;

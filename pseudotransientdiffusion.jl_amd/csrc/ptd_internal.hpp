@@ -424,12 +424,14 @@ bool fused_available(const ptd_ctx* c);
 bool fused2_available(const ptd_ctx* c);
 // mixed precision: float32 correction sweeps (1 or 2 iterations), source and correction kernels
 hipError_t launch_delta_sweep(ptd_ctx* c, bool two, bool check);
-hipError_t launch_make_source(ptd_ctx* c);      // src = float(ResH * dt) on inner cells
-hipError_t launch_add_correction(ptd_ctx* c);   // H[hcur] += double(delta)
 hipError_t launch_f64_to_f32(ptd_ctx* c, float* dst, const double* src, int64_t n);
 // correction fluxes <-> true fluxes: to_p: p = float(q + g(H)); !to_p: q = double(p) - g(H), where g(H)
 // is the equilibrium flux magnitude coef*dH/h of the float64 iterate H on each face
 hipError_t launch_flux_state(ptd_ctx* c, bool to_p);
+// fused correction boundaries: residual + source + float32(Hc) + delta = 0 (partials -> dred), and
+// p <- p + g(Hc + delta) - g(Hc) with Hc + delta into the other H buffer (flips hcur)
+hipError_t launch_outer_residual(ptd_ctx* c);
+hipError_t launch_outer_update(ptd_ctx* c);
 
 // gradients of the lin / step solves (ptd_grad.hip): adjoint source (H - Hobs)*dt with L's partials
 // reduced into c->dred, and one step's gradient contribution

@@ -1047,42 +1047,10 @@ bool fused2_available(const ptd_ctx* c) { return c->p.ndim == 3 && fused_availab
 // mixed precision: float32 correction sweeps around a float64 iterate
 // ---------------------------------------------------------------------------------------------
 
-// src = float(R * dt) on the inner cells (R = ResH of the float64 iterate).
-__global__ void __launch_bounds__(kBlock)
-k_make_source(const KP<double> p, const double* __restrict__ resh, float* __restrict__ src, double dt, int64_t count) {
-    const int64_t t = (int64_t)blockIdx.x * kBlock + threadIdx.x;
-    if (t >= count) return;
-    const CellIdx c = cell_index(p, t);
-    src[c.h] = (float)(resh[t] * dt);
-}
-
-// H += double(delta) on the inner cells.
-__global__ void __launch_bounds__(kBlock)
-k_add_correction(const KP<double> p, double* __restrict__ H, const float* __restrict__ dl, int64_t count) {
-    const int64_t t = (int64_t)blockIdx.x * kBlock + threadIdx.x;
-    if (t >= count) return;
-    const CellIdx c = cell_index(p, t);
-    H[c.h] += (double)dl[c.h];
-}
-
 __global__ void __launch_bounds__(kBlock)
 k_f64_to_f32(float* __restrict__ dst, const double* __restrict__ src, int64_t n) {
     const int64_t t = (int64_t)blockIdx.x * kBlock + threadIdx.x;
     if (t < n) dst[t] = (float)src[t];
-}
-
-hipError_t launch_make_source(ptd_ctx* c) {
-    const KP<double> kp = make_kp<double>(c->p);
-    hipLaunchKernelGGL(k_make_source, dim3(grid_for(c->nR)), dim3(kBlock), 0, c->stream, kp,
-                       (const double*)c->resh, (float*)c->src, c->p.dt, c->nR);
-    return hipGetLastError();
-}
-
-hipError_t launch_add_correction(ptd_ctx* c) {
-    const KP<double> kp = make_kp<double>(c->p);
-    hipLaunchKernelGGL(k_add_correction, dim3(grid_for(c->nR)), dim3(kBlock), 0, c->stream, kp,
-                       (double*)c->H[c->hcur], (const float*)c->dl[c->dcur], c->nR);
-    return hipGetLastError();
 }
 
 hipError_t launch_f64_to_f32(ptd_ctx* c, float* dst, const double* src, int64_t n) {
@@ -1154,6 +1122,128 @@ hipError_t launch_flux_state(ptd_ctx* c, bool to_p) {
     if (c->p.dcase == PTD_NONLIN) return to_p ? flux_state_t<PTD_NONLIN, true>(c) : flux_state_t<PTD_NONLIN, false>(c);
     if (c->p.dcase == PTD_STEP) return to_p ? flux_state_t<PTD_STEP, true>(c) : flux_state_t<PTD_STEP, false>(c);
     return to_p ? flux_state_t<PTD_LIN, true>(c) : flux_state_t<PTD_LIN, false>(c);
+}
+
+// Start of a correction (one pass over every cell of the float64 iterate Hc): the consistent residual
+// R of the float64 system on the inner cells (residual fluxes from Hc itself, the same expressions as
+// compute_flux_res! + check_res!, scripts/diff_3D/diff_3D_nonlin_perf.jl:44-53 in the context's form),
+// written to ResH, with its per-block (sum r^2, max |r|); the float32 source src = R*dt; float32(Hc) for
+// the correction sweep's aux channel (D = H^3); and delta = 0.
+template <int DC, int FORM>
+__global__ void __launch_bounds__(kBlock)
+k_outer_residual(const KP<double> p, const double* __restrict__ H, const double* __restrict__ Hold,
+                 const double* __restrict__ D, double* __restrict__ resh, float* __restrict__ src,
+                 float* __restrict__ hcf, float* __restrict__ dl, double2* __restrict__ partial, int64_t n) {
+    using Ph = Phys<double, DC, FORM, kExact>;
+    const int64_t g = (int64_t)blockIdx.x * kBlock + threadIdx.x;
+    double s = 0.0, m = 0.0;
+    if (g < n) {
+        const double Hc = H[g];
+        if (hcf) hcf[g] = (float)Hc;
+        dl[g] = 0.0f;
+        const int64_t nx = p.nx, ny = p.ny, pl = nx * ny;
+        const int64_t i = g % nx, rest = g / nx, j = rest % ny, k = rest / ny;
+        if (i >= 1 && i <= nx - 2 && j >= 1 && j <= ny - 2 && k >= 1 && k <= p.nz - 2) {
+            auto Dv = [&](int64_t o) { return DC == PTD_STEP ? D[o] : 0.0; };
+            const double Dc = Dv(g);
+            const double fxw = Ph::face_res(p, H[g - 1], Hc, Dv(g - 1), Dc, p.dx, p._dx);
+            const double fxe = Ph::face_res(p, Hc, H[g + 1], Dc, Dv(g + 1), p.dx, p._dx);
+            const double fys = Ph::face_res(p, H[g - nx], Hc, Dv(g - nx), Dc, p.dy, p._dy);
+            const double fyn = Ph::face_res(p, Hc, H[g + nx], Dc, Dv(g + nx), p.dy, p._dy);
+            const double fzd = Ph::face_res(p, H[g - pl], Hc, Dv(g - pl), Dc, p.dz, p._dz);
+            const double fzu = Ph::face_res(p, Hc, H[g + pl], Dc, Dv(g + pl), p.dz, p._dz);
+            const double dv = divergence<double, FORM, 3>(p, fxe - fxw, fyn - fys, fzu - fzd);
+            const double r = residual<double, FORM>(p, Hc, Hold[g], dv);
+            resh[(i - 1) + p.mx * ((j - 1) + p.my * (k - 1))] = r;
+            src[g] = (float)(r * p.dt);
+            s = r * r;
+            m = fabs(r);
+        } else {
+            src[g] = 0.0f;
+        }
+    }
+    block_sum_max(s, m, threadIdx.x, kBlock, &partial[blockIdx.x]);
+}
+
+// End of a correction (one pass over every cell): the correction fluxes move from Hc to Hc + delta,
+// p <- float((double(p) - g(Hc)) + g(Hc + delta)) on each face (the face whose upper cell is this cell;
+// g = coef * dH / h the equilibrium flux magnitude, as in k_flux_state), and Hc + delta goes to the
+// other H buffer (inner cells; the boundary cells hold the same values in both buffers).
+template <int DC>
+__global__ void __launch_bounds__(kBlock)
+k_outer_update(const KP<double> p, const double* __restrict__ Hin, double* __restrict__ Hout,
+               const float* __restrict__ dl, const double* __restrict__ D, float* __restrict__ px,
+               float* __restrict__ py, float* __restrict__ pz, int64_t n) {
+    const int64_t g = (int64_t)blockIdx.x * kBlock + threadIdx.x;
+    if (g >= n) return;
+    const int64_t nx = p.nx, ny = p.ny, nz = p.nz, pl = nx * ny;
+    const int64_t i = g % nx, rest = g / nx, j = rest % ny, k = rest / ny;
+    const bool ii = i >= 1 && i <= nx - 2, ji = j >= 1 && j <= ny - 2, ki = k >= 1 && k <= nz - 2;
+    auto gflux = [&](double Ha, double Hb, int64_t a, int64_t b, double ih) {
+        double coef;
+        if constexpr (DC == PTD_NONLIN) {
+            const double hf = 0.5 * (Ha + Hb);
+            coef = hf * hf * hf;
+        } else if constexpr (DC == PTD_STEP) {
+            coef = (D[a] + D[b]) * 0.5;
+        } else {
+            coef = p.d0;
+        }
+        return coef * (ih * (Hb - Ha));
+    };
+    auto move = [&](float* pf, int64_t f, int64_t a, double ih) {
+        const double Ha = Hin[a], Hb = Hin[g];
+        const double go = gflux(Ha, Hb, a, g, ih);
+        const double gn = gflux(Ha + (double)dl[a], Hb + (double)dl[g], a, g, ih);
+        pf[f] = (float)(((double)pf[f] - go) + gn);
+    };
+    if (i >= 1 && ji && ki) move(px, (i - 1) + (nx - 1) * ((j - 1) + (ny - 2) * (k - 1)), g - 1, p._dx);
+    if (j >= 1 && ii && ki) move(py, (i - 1) + (nx - 2) * ((j - 1) + (ny - 1) * (k - 1)), g - nx, p._dy);
+    if (k >= 1 && ii && ji) move(pz, (i - 1) + (nx - 2) * ((j - 1) + (ny - 2) * (k - 1)), g - pl, p._dz);
+    if (ii && ji && ki) Hout[g] = Hin[g] + (double)dl[g];
+}
+
+template <int DC, int FORM>
+static hipError_t outer_residual_t(ptd_ctx* c) {
+    const KP<double> kp = make_kp<double>(c->p);
+    const unsigned gb = grid_for(c->nH);
+    if ((int64_t)gb > c->npartial) return hipErrorInvalidValue;
+    hipLaunchKernelGGL((k_outer_residual<DC, FORM>), dim3(gb), dim3(kBlock), 0, c->stream, kp,
+                       (const double*)c->H[c->hcur], (const double*)c->Hold, (const double*)c->D, (double*)c->resh,
+                       (float*)c->src, (float*)c->hcf, (float*)c->dl[c->dcur], (double2*)c->partial, c->nH);
+    hipLaunchKernelGGL(k_reduce_partials, dim3(1), dim3(kBlock), 0, c->stream, (const double2*)c->partial,
+                       (int64_t)gb, c->dred);
+    return hipGetLastError();
+}
+
+hipError_t launch_outer_residual(ptd_ctx* c) {
+    const bool perf = c->p.form == PTD_PERF;
+    switch (c->p.dcase) {
+        case PTD_LIN: return perf ? outer_residual_t<PTD_LIN, PTD_PERF>(c) : outer_residual_t<PTD_LIN, PTD_PLAIN>(c);
+        case PTD_STEP: return perf ? outer_residual_t<PTD_STEP, PTD_PERF>(c) : outer_residual_t<PTD_STEP, PTD_PLAIN>(c);
+        default: return perf ? outer_residual_t<PTD_NONLIN, PTD_PERF>(c) : outer_residual_t<PTD_NONLIN, PTD_PLAIN>(c);
+    }
+}
+
+hipError_t launch_outer_update(ptd_ctx* c) {
+    const KP<double> kp = make_kp<double>(c->p);
+    const dim3 g(grid_for(c->nH)), bl(kBlock);
+    const double* Hin = (const double*)c->H[c->hcur];
+    double* Hout = (double*)c->H[1 - c->hcur];
+    const float* dl = (const float*)c->dl[c->dcur];
+    const double* D = (const double*)c->D;
+    float* px = (float*)c->pq[0][c->pcur];
+    float* py = (float*)c->pq[1][c->pcur];
+    float* pz = (float*)c->pq[2][c->pcur];
+    if (c->p.dcase == PTD_NONLIN)
+        hipLaunchKernelGGL((k_outer_update<PTD_NONLIN>), g, bl, 0, c->stream, kp, Hin, Hout, dl, D, px, py, pz, c->nH);
+    else if (c->p.dcase == PTD_STEP)
+        hipLaunchKernelGGL((k_outer_update<PTD_STEP>), g, bl, 0, c->stream, kp, Hin, Hout, dl, D, px, py, pz, c->nH);
+    else
+        hipLaunchKernelGGL((k_outer_update<PTD_LIN>), g, bl, 0, c->stream, kp, Hin, Hout, dl, D, px, py, pz, c->nH);
+    hipError_t e = hipGetLastError();
+    if (e == hipSuccess) c->hcur = 1 - c->hcur;
+    return e;
 }
 
 // One or two float32 correction iterations on (delta, correction fluxes) with source src = R*dt in
