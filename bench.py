@@ -8,8 +8,15 @@ value = T_eff = A_eff / t_it with A_eff = (2*4+2)*nx*ny*nz*8 bytes per iteration
 definition (scripts/diff_3D/diff_3D_nonlin_perf.jl:114-116). K iterations are timed with HIP events on
 the library's stream, bracketed by device synchronisation.
 
-Extra legs (rank 0, N=1): time-to-solution of the full converging 5-step solve (tol 1e-8, check every
-nout=10 iterations) on the GPU, and the CPU oracle timed on the host cores on a bounded sample.
+Extra legs (N=1), each in its own object of the JSON line:
+ - time_to_solution: the full converging 5-step solve (tol 1e-8, check every nout=10 iterations) in
+   float64 and with float32 correction sweeps inside float64 defect correction (--tts-n 1023 for the
+   precision workload's grid);
+ - batch: 96 independent 2D / 3D solves of uneven sizes and D types in one batch (aggregate T_eff,
+   p50 / p99 latency);
+ - gradient: dL/dD of the 3D step case at 255^3 over 10 steps by the adjoint, checked by central
+   differences on 20 directions;
+ - cpu_baseline: the CPU oracle timed on the host cores on a bounded sample.
 """
 import argparse
 import json
@@ -209,12 +216,12 @@ def main():
     ap.add_argument("--cpu-seconds", type=float, default=15.0)
     ap.add_argument("--tts-n", type=int, default=0,
                     help="grid size of the time-to-solution legs (default: --n; the precision workload is 1023)")
-    ap.add_argument("--batch", type=int, default=0,
-                    help="also solve the batched workload with this many members (96: 2D {128..1024}^2 and "
-                         "3D {64..256}^3 grids, lin / step / H^3, 5 physical steps each)")
-    ap.add_argument("--grad", type=int, default=0,
-                    help="also run the gradient workload at this grid size (255: 3D step case, 10 physical "
-                         "steps, dL/dD by the adjoint, checked by central differences)")
+    ap.add_argument("--batch", type=int, default=96,
+                    help="members of the batched workload leg (2D {128..1024}^2 and 3D {64..256}^3 grids, "
+                         "lin / step / H^3, 5 physical steps each); 0 skips it")
+    ap.add_argument("--grad", type=int, default=255,
+                    help="grid of the gradient leg (3D step case, 10 physical steps, dL/dD by the adjoint, "
+                         "checked by central differences); 0 skips it")
     ap.add_argument("--grad-fd", type=int, default=20, help="finite-difference directions of the gradient check")
     ap.add_argument("--calib-copy", action="store_true",
                     help="only run the 1 GiB copy used to scale rocprofv3 FETCH_SIZE/WRITE_SIZE, then exit")
@@ -304,6 +311,12 @@ def main():
         "roofline": {"bound": "hbm", "achieved": round(t_eff, 2), "peak": PEAK_HBM_GBS, "unit": "GB/s",
                      "frac": round(t_eff / PEAK_HBM_GBS, 4), "traffic": traffic},
     }
+    if traffic:
+        # achieved counts the reference's A_eff for the launch's two iterations; the measured HBM bytes
+        # of the launch, at the same launch time, give the actual HBM rate
+        launch_s = t_it * iters_per_launch
+        out["roofline"]["traffic_GBs"] = round(traffic / launch_s / 1e9, 2)
+        out["roofline"]["traffic_frac"] = round(traffic / launch_s / 1e9 / PEAK_HBM_GBS, 4)
 
     # the boundary takes host arrays: the same K iterations including the upload of H, Hold and the
     # download of H over PCIe (reported beside `value`, which starts with the fields resident in HBM)
