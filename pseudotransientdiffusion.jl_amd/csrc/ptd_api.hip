@@ -62,6 +62,8 @@ int check_params(const ptd_params* p) {
     const int64_t ny = p->ndim >= 2 ? p->ny : 1, nz = p->ndim >= 3 ? p->nz : 1;
     const double cells = (double)p->nx * (double)ny * (double)nz;
     PTD_REQUIRE(cells < 9.0e18 / 8.0, PTD_ERR_ARG, "grid too large");
+    PTD_REQUIRE(p->precision != PTD_MIXED || ptd::max_chunk_tz(*p, 8) >= 1, PTD_ERR_ARG,
+                "PTD_MIXED: an x-y plane of the grid must stay under 4 GiB / 6");
     return PTD_OK;
 }
 
@@ -252,17 +254,20 @@ int run_delta(ptd_ctx* c, int64_t n, bool check_last) {
 // Defect-correction PT solve (PTD_MIXED). Outer loop in float64: R = residual of the backward-Euler
 // system at the iterate Hc (flux from Hc itself); inner loop in float32: PT sweeps on the correction
 // delta with source R (same Re / theta / dtau rules, evaluated at Hc + delta), checked on the float32
-// correction residual every nout iterations once a predicted minimum has run; then Hc += delta. The inner sweeps are the reference
-// PT iteration written in shifted variables (delta = H - Hc, p = q + g(Hc)); they start from the true
-// fluxes q carried over from the previous solve, as the reference's q arrays are (starting from the
-// equilibrium flux of a non-equilibrium H instead overshoots). The inner solve stops once it has
-// reduced its residual kInner-fold or reached 0.5*tol, or stalls. The float32 floor of one correction
-// solve (rounding of delta, amplified by D/dx^2) lies ~3.5 decades below its starting residual, and a
-// restart from floor noise converges markedly slower than the float64 iteration; two decades per
-// correction keep every restart clear of it (measured at 511^3: 4070 sweeps for the 5-step solve vs
-// 3860 float64 iterations; 4630 with 1e-5).
+// correction residual; then Hc += delta. The inner sweeps are the reference PT iteration written in
+// shifted variables (delta = H - Hc, p = q + g(Hc)); they start from the true fluxes q carried over from
+// the previous solve, as the reference's q arrays are (starting from the equilibrium flux of a
+// non-equilibrium H instead overshoots), and the flux state moves with Hc at every correction, so the
+// corrections continue one PT iteration. An inner solve stops once it has reduced its residual
+// kInner-fold, or reached kLast*tol (the last correction), or stalls. The float32 floor of one
+// correction solve (rounding of delta, amplified by D/dx^2) lies ~3.5 decades below its starting
+// residual, and a restart from floor noise converges markedly slower than the float64 iteration; two
+// decades per correction keep every restart clear of it. Checks are placed by prediction: the first
+// after 70% of the sweeps the last complete correction took (scaled by the decades to go), the next
+// ones after the sweeps the observed decay rate predicts. Measured at 511^3 (5 steps): 3940-3970
+// float32 sweeps and ~70-95 checks, vs 3860 float64 iterations checked every 10.
 int pt_solve_mixed(ptd_ctx* c, int64_t* iters, double* err_rms, double* err_max) {
-    constexpr double kInner = 1e-2;
+    constexpr double kInner = 1e-2, kLast = 0.85;
     const ptd_params& p = c->p;
     if (p.dcase == PTD_STEP) PTD_HIP(ptd::launch_f64_to_f32(c, (float*)c->Df, (const double*)c->D, c->nH));
     // residual of the float64 system at Hc, with the next correction's source, float32(Hc), delta = 0
@@ -282,24 +287,30 @@ int pt_solve_mixed(ptd_ctx* c, int64_t* iters, double* err_rms, double* err_max)
             PTD_HIP(ptd::launch_flux_state(c, true));
             in_p = true;
         }
-        const double target = std::max(0.5 * p.tol, kInner * err);
+        // the last correction of the solve aims just below tol: the float64 residual after the update
+        // lands within a few percent of the float32 one
+        const bool last = kInner * err <= kLast * p.tol;
+        const double target = last ? kLast * p.tol : kInner * err;
         double best = HUGE_VAL;
         int stall = 0;
         int64_t it = 0;
         // A correction solve takes about as many sweeps per decade of reduction as the last one that
-        // met the kInner target: the first check comes after 60% of the predicted count (in whole nout
-        // blocks), then every nout sweeps. (These checks are the float32 inner solve's own; the float64
-        // stopping test above is evaluated after every correction.)
+        // met the kInner target: the first check comes after 70% of the predicted count (in whole nout
+        // blocks). (These checks are the float32 inner solve's own; the float64 stopping test above is
+        // evaluated after every correction.)
         const double decades = std::log10(err / target) / std::log10(1.0 / kInner);
-        const int64_t first = (int64_t)(0.6 * (double)c->inner_hint * std::min(1.0, decades)) / p.nout * p.nout;
+        const int64_t first = (int64_t)(0.7 * (double)c->inner_hint * std::min(1.0, decades)) / p.nout * p.nout;
         if (first > 0 && total + first < p.itmax) {
             st = run_delta(c, first, false);
             if (st != PTD_OK) return st;
             it = first;
         }
         bool met = false;
+        int64_t gap = p.nout;             // sweeps before the next check
+        double r_prev = err;
+        int64_t it_prev = 0;
         while (total + it < p.itmax) {
-            const int64_t n = std::min<int64_t>(p.nout, p.itmax - total - it);
+            const int64_t n = std::min<int64_t>(gap, p.itmax - total - it);
             st = run_delta(c, n, true);
             if (st != PTD_OK) return st;
             it += n;
@@ -309,7 +320,7 @@ int pt_solve_mixed(ptd_ctx* c, int64_t* iters, double* err_rms, double* err_max)
             const double r = p.stop_norm == PTD_NORM_MAX ? imx : irms;
             if (std::isnan(r)) break;
             if (r <= target) {
-                met = target > 0.5 * p.tol;
+                met = !last;
                 break;
             }
             if (r < 0.95 * best) {
@@ -318,6 +329,17 @@ int pt_solve_mixed(ptd_ctx* c, int64_t* iters, double* err_rms, double* err_max)
             } else if (++stall >= 3) {
                 break;  // at the float32 floor of this correction
             }
+            // next check: the sweeps the decay rate since the previous check predicts for the rest of the
+            // reduction (80% of them for the last correction, whose overshoot is not used), at most half
+            // the sweeps run so far (whole nout blocks, at least one)
+            gap = p.nout;
+            if (r < r_prev && it > it_prev) {
+                const double per_sweep = std::log(r / r_prev) / (double)(it - it_prev);
+                const double need = std::min((last ? 0.8 : 1.0) * std::log(target / r) / per_sweep, 0.5 * (double)it);
+                gap = std::max<int64_t>(p.nout, (int64_t)need / p.nout * p.nout);
+            }
+            r_prev = r;
+            it_prev = it;
         }
         total += it;
         if (met) c->inner_hint = it;

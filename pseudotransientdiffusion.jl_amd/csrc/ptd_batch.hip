@@ -232,6 +232,8 @@ int32_t ptd_batch_create(const ptd_params* params, int32_t n, int32_t device, pt
         if (p.nout != params[0].nout || p.nout % 2 != 0)
             return ptd::set_error(PTD_ERR_ARG, who + "nout must be even and the same for every member");
         if (p.itmax % p.nout != 0) return ptd::set_error(PTD_ERR_ARG, who + "itmax must be a multiple of nout");
+        if (p.ndim == 3 && ptd::max_chunk_tz(p, 8) < 32)
+            return ptd::set_error(PTD_ERR_ARG, who + "x-y plane too large for the batched 3D sweep (37 planes under 4 GiB)");
     }
     int ndev = 0;
     if (hipGetDeviceCount(&ndev) != hipSuccess || device < 0 || device >= ndev)

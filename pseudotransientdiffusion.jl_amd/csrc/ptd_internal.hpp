@@ -32,6 +32,7 @@ struct KP {
     T K;                  // max_l2 * _dt
     T K8;                 // 8 * K (nonlinear faces work with u = (Ha + Hb)^3 = 8 * Hface^3)
     T _dx8, _dy8, _dz8;   // _dx / 8, _dy / 8, _dz / 8
+    T _dxs, _dys, _dzs;   // _dx * _dx / 8 ...: divergence weights of the scaled correction fluxes (kDelta)
     T c1;                 // max_l / Vpdtau * Resc           (theta = c1 / Re)
     T c2;                 // Vpdtau * max_l * Resc           (nonlinear dtau_rho = c2 / (H^3 Re))
     T c3;                 // Vpdtau * max_l / Re             (step dtau_rho = c3 / maxloc(D))
@@ -74,6 +75,9 @@ KP<T> make_kp(const ptd_params& p) {
     k._dx8 = (T)(0.125 / p.dx);
     k._dy8 = (T)(k.hy ? 0.125 / p.dy : 0.0);
     k._dz8 = (T)(k.hz ? 0.125 / p.dz : 0.0);
+    k._dxs = (T)(0.125 / (p.dx * p.dx));
+    k._dys = (T)(k.hy ? 0.125 / (p.dy * p.dy) : 0.0);
+    k._dzs = (T)(k.hz ? 0.125 / (p.dz * p.dz) : 0.0);
     k.c1 = (T)(p.max_l / p.vpdt * p.resc);
     k.c2 = (T)(p.vpdt * p.max_l * p.resc);
     k.c3 = (T)(p.re > 0 ? p.vpdt * p.max_l / p.re : 0.0);
@@ -207,6 +211,8 @@ __device__ __forceinline__ double rsq_1nr(double x) {
     return fma(0.5 * y, fma(-(x * y), y, 1.0), y);
 }
 __device__ __forceinline__ float rsq_1nr(float x) { return __builtin_amdgcn_rsqf(x); }
+// hardware square root (about 1 ulp), for coefficients whose rounding only affects the convergence rate
+__device__ __forceinline__ float sqrt_hw(float x) { return __builtin_amdgcn_sqrtf(x); }
 
 template <typename T>
 __device__ __forceinline__ T fmaT(T a, T b, T c) { return fma(a, b, c); }
@@ -245,7 +251,14 @@ enum { kExact = PTD_ARITH_EXACT, kFast = PTD_ARITH_FAST };
 // correction delta (float32) around the float64 iterate Hc, whose float32 rounding travels through the
 // kernels in the per-cell "aux" channel (the channel the step case uses for D). The coefficients only
 // need float32 accuracy: the fixed point of the defect correction is set by the float64 residual.
+// Its correction fluxes are stored scaled by 8*h of their axis (P = 8h*p), which takes the 1/(8h) of
+// the face flux out of every face evaluation and into the divergence weights _dxs, _dys, _dzs.
 constexpr int kDelta = 3;
+// scale of the stored correction fluxes of axis a for D case DC (kDelta: 8*h; others: 1)
+inline double corr_flux_scale(const ptd_params& p, int dc, int axis) {
+    if (dc != PTD_NONLIN) return 1.0;
+    return 8.0 * (axis == 0 ? p.dx : axis == 1 ? p.dy : p.dz);
+}
 
 template <typename T, int DC>
 struct AuxOf {
@@ -268,12 +281,13 @@ struct Phys {
             // correction flux around Hc: g(Hc+delta) - g(Hc) = dD*dHc + (hbar+dbar)^3*ddelta with
             // dD = (hbar+dbar)^3 - hbar^3 = dbar*(3*hbar*(hbar+dbar) + dbar^2), no cancellation. In the
             // sums S = 2*hbar, Tt = 2*dbar, U = S + Tt: dD = Tt*(3*S*U + Tt^2)/8, (hbar+dbar)^3 = U^3/8.
+            // q, g and the returned flux are the scaled ones (8h times the flux).
             const T S = Da + Db;
             const T Tt = Ha + Hb;
             const T U = S + Tt;
             const T u = floor_h3x8(U * U * U);
             const T dHc = Db - Da;
-            const T g = ih8 * fmaT(u, dH, Tt * fmaT((T)3 * S, U, Tt * Tt) * dHc);
+            const T g = fmaT(u, dH, Tt * fmaT((T)3 * S, U, Tt * Tt) * dHc);
             const T re = re_fast8(p, u);
             if constexpr (Q2) q2 = -g;
             return fmaT(-(q + g), re * rcp_1nr(re + p.c1), q);
@@ -325,6 +339,11 @@ struct Phys {
     __device__ static __forceinline__ T div(const KP<T>& p, T ddx, T ddy, T ddz) {
         if constexpr (ARITH == kExact && DC != kDelta) {
             return divergence<T, FORM, NDIM>(p, ddx, ddy, ddz);
+        } else if constexpr (DC == kDelta) {
+            T s = p._dxs * ddx;
+            if constexpr (NDIM >= 2) s = fmaT(p._dys, ddy, s);
+            if constexpr (NDIM >= 3) s = fmaT(p._dzs, ddz, s);
+            return s;
         } else {
             T s = p._dx * ddx;
             if constexpr (NDIM >= 2) s = fmaT(p._dy, ddy, s);
@@ -338,10 +357,11 @@ struct Phys {
     __device__ static __forceinline__ T cell(const KP<T>& p, T H, T Hold, T div, T dmax, Ta Ac = Ta{}) {
         if constexpr (DC == kDelta) {
             // correction update with dtau from H = Hc + delta; Hold carries R*dt (source R)
+            // P = h3*Re = pi*h3 + sqrt(h3*(pi^2*h3 + K))
             const T src = fmaT(p._dt, Hold, -div);
             const T Hf = Ac + H;
             const T h3 = floor_h3(Hf * Hf * Hf);
-            const T P = h3 * re_fast(p, h3);
+            const T P = fmaT(p.pi, h3, sqrt_hw(h3 * fmaT(p.pi2, h3, p.K)));
             return fmaT(fmaT(-H, p._dt, src), p.c2 * rcp_1nr(fmaT(p.c2, p._dt, P)), H);
         } else if constexpr (ARITH == kExact) {
             T dtr;
@@ -365,6 +385,49 @@ struct Phys {
                 return fmaT(fmaT(-H, p._dt, src), p.c2 * rcp_1nr(fmaT(p.c2, p._dt, P)), H);
             }
         }
+    }
+
+    // kDelta with damping carried between the two levels of the two-level sweep: the level-1 face /
+    // cell computes its damping factor (w, v) and returns it; the level-2 evaluation of the same face /
+    // cell one iteration later reuses it. The damping factors only set the convergence rate of the
+    // correction sweep (its fixed point q = -g, residual = 0 does not depend on them).
+    __device__ static __forceinline__ T face_w(const KP<T>& p, T q, T Ha, T Hb, Ta Da, Ta Db, T& w) {
+        static_assert(DC == kDelta, "damping carry is a correction-sweep rule");
+        const T dH = Hb - Ha;
+        const T S = Da + Db;
+        const T Tt = Ha + Hb;
+        const T U = S + Tt;
+        const T u = floor_h3x8(U * U * U);
+        const T dHc = Db - Da;
+        const T g = fmaT(u, dH, Tt * fmaT((T)3 * S, U, Tt * Tt) * dHc);
+        const T re = re_fast8(p, u);
+        w = re * rcp_1nr(re + p.c1);
+        return fmaT(-(q + g), w, q);
+    }
+    template <bool Q2>
+    __device__ static __forceinline__ T face_lag(T q, T Ha, T Hb, Ta Da, Ta Db, T w, T& q2) {
+        static_assert(DC == kDelta, "damping carry is a correction-sweep rule");
+        const T dH = Hb - Ha;
+        const T S = Da + Db;
+        const T Tt = Ha + Hb;
+        const T U = S + Tt;
+        const T dHc = Db - Da;
+        const T g = fmaT(U * U * U, dH, Tt * fmaT((T)3 * S, U, Tt * Tt) * dHc);
+        if constexpr (Q2) q2 = -g;
+        return fmaT(-(q + g), w, q);
+    }
+    __device__ static __forceinline__ T cell_w(const KP<T>& p, T H, T Hold, T div, Ta Ac, T& v) {
+        static_assert(DC == kDelta, "damping carry is a correction-sweep rule");
+        const T src = fmaT(p._dt, Hold, -div);
+        const T Hf = Ac + H;
+        const T h3 = floor_h3(Hf * Hf * Hf);
+        const T P = fmaT(p.pi, h3, sqrt_hw(h3 * fmaT(p.pi2, h3, p.K)));
+        v = p.c2 * rcp_1nr(fmaT(p.c2, p._dt, P));
+        return fmaT(fmaT(-H, p._dt, src), v, H);
+    }
+    __device__ static __forceinline__ T cell_lag(const KP<T>& p, T H, T Hold, T div, T v) {
+        const T src = fmaT(p._dt, Hold, -div);
+        return fmaT(fmaT(-H, p._dt, src), v, H);
     }
 
     __device__ static __forceinline__ T resid(const KP<T>& p, T H, T Hold, T div) {
@@ -420,6 +483,7 @@ hipError_t launch_iter_params(ptd_ctx* c);
 hipError_t launch_residual(ptd_ctx* c, bool write_resh);  // partial sums -> dred
 hipError_t launch_fused(ptd_ctx* c, bool check);          // one fused PT iteration (2D / 3D)
 hipError_t launch_fused2(ptd_ctx* c, bool check);         // two fused PT iterations in one pass (3D)
+int64_t max_chunk_tz(const ptd_params& p, int64_t esz);
 bool fused_available(const ptd_ctx* c);
 bool fused2_available(const ptd_ctx* c);
 // mixed precision: float32 correction sweeps (1 or 2 iterations), source and correction kernels

@@ -234,6 +234,17 @@ template <typename T>
 __device__ __forceinline__ void stp(T* __restrict__ plane, uint32_t off, T v) {
     *reinterpret_cast<T*>(reinterpret_cast<char*>(plane) + off) = v;
 }
+// The same from a block-uniform chunk base and a 32-bit byte offset that already holds the plane: the
+// address is (scalar base) + (vector offset), one global access with a scalar base and no 64-bit
+// address arithmetic or per-array 64-bit address registers.
+template <typename T>
+__device__ __forceinline__ T ldc(const char* __restrict__ base, uint32_t off) {
+    return *reinterpret_cast<const T*>(base + off);
+}
+template <typename T>
+__device__ __forceinline__ void stc(char* __restrict__ base, uint32_t off, T v) {
+    *reinterpret_cast<T*>(base + off) = v;
+}
 
 // Lane shuffles by one lane (x neighbours).
 template <typename V>
@@ -538,7 +549,7 @@ constexpr bool t2_compact() {
 }
 template <typename T, int DC, int ARITH, int R, bool CHECK>
 constexpr int t2_waves_per_simd() {
-    return sizeof(T) == 4 ? 8 : (t2_compact<T, DC, ARITH, CHECK>() ? 2 * (R / 4) : 1);
+    return sizeof(T) == 4 ? 2 * (R / 4) : (t2_compact<T, DC, ARITH, CHECK>() ? 2 * (R / 4) : 1);
 }
 template <typename T>
 constexpr int t2_prefetch_depth() {
@@ -546,7 +557,7 @@ constexpr int t2_prefetch_depth() {
 }
 template <typename T, int DC, int ARITH, bool CHECK>
 constexpr int t2_rows() {
-    return t2_compact<T, DC, ARITH, CHECK>() ? 12 : 16;
+    return t2_compact<T, DC, ARITH, CHECK>() || (sizeof(T) == 4 && CHECK) ? 12 : 16;
 }
 constexpr int kMinRows2 = 12;  // smallest two-level block (sizes the partials buffer)
 
@@ -617,38 +628,58 @@ __device__ __forceinline__ void fused3d_t2_tile(const KP<T>& p, const T* __restr
     const uint32_t oQy = (uint32_t)((iyq - 1) + (nx - 2) * (jyq - 1)) * es;
     const uint32_t oQz = (uint32_t)((iyq - 1) + (nx - 2) * (jzq - 1)) * es;
 
+    // Every plane this tile touches lies in kb .. kb + tz + 4 (kb = ka - 1, or 0): the arrays are addressed
+    // from their plane kb (block-uniform bases) with 32-bit byte offsets (the host keeps (tz + 5) planes
+    // of H under 4 GiB).
+    const int kb = ka > 0 ? ka - 1 : 0;
+    const char* const bHin = reinterpret_cast<const char*>(Hin + psH * kb);
+    const char* const bD = reinterpret_cast<const char*>(D + (AUX ? psH * kb : 0));
+    const char* const bHo = reinterpret_cast<const char*>(Hold + psH * kb);
+    const char* const bqx = reinterpret_cast<const char*>(qxi + psQx * kb);
+    const char* const bqy = reinterpret_cast<const char*>(qyi + psQy * kb);
+    const char* const bqz = reinterpret_cast<const char*>(qzi + psQz * kb);
+    char* const bHout = reinterpret_cast<char*>(Hout + psH * kb);
+    char* const bqxo = reinterpret_cast<char*>(qxo + psQx * kb);
+    char* const bqyo = reinterpret_cast<char*>(qyo + psQy * kb);
+    char* const bqzo = reinterpret_cast<char*>(qzo + psQz * kb);
+    const uint32_t pbH = (uint32_t)psH * es, pbA = (uint32_t)psH * (uint32_t)sizeof(Ta);
+    const uint32_t pbQx = (uint32_t)psQx * es, pbQy = (uint32_t)psQy * es, pbQz = (uint32_t)psQz * es;
+
     auto load_plane = [&](int pl) {  // level-1 operands of plane pl, pl in [0, nz-1]
         PlaneOps2<T, Ta> o;
-        const int ph = pl + 1 < nz ? pl + 1 : nz - 1;
-        const int pq = pl - 1 < 0 ? 0 : (pl - 1 > nz - 3 ? nz - 3 : pl - 1);
-        const int pz = pl > nz - 2 ? nz - 2 : pl;
-        o.Hn = ldp(Hin + psH * ph, oH, true);
-        if constexpr (AUX) o.Dn = ldp(D + psH * ph, oA, true);
+        const uint32_t ph = (uint32_t)((pl + 1 < nz ? pl + 1 : nz - 1) - kb);
+        const uint32_t pq = (uint32_t)((pl - 1 < 0 ? 0 : (pl - 1 > nz - 3 ? nz - 3 : pl - 1)) - kb);
+        const uint32_t pz = (uint32_t)((pl > nz - 2 ? nz - 2 : pl) - kb);
+        o.Hn = ldc<T>(bHin, oH + ph * pbH);
+        if constexpr (AUX) o.Dn = ldc<Ta>(bD, oA + ph * pbA);
         else o.Dn = Ta{};
-        o.Ho = ldp(Hold + psH * pl, oH, true);
-        o.qx = ldp(qxi + psQx * pq, oQx, true);
-        o.qy = ldp(qyi + psQy * pq, oQy, true);
-        o.qz = ldp(qzi + psQz * pz, oQz, true);
+        o.Ho = ldc<T>(bHo, oH + (uint32_t)(pl - kb) * pbH);
+        o.qx = ldc<T>(bqx, oQx + pq * pbQx);
+        o.qy = ldc<T>(bqy, oQy + pq * pbQy);
+        o.qz = ldc<T>(bqz, oQz + pz * pbQz);
         return o;
     };
 
     // level-1 lower z face of plane ka (face (ka-1 | ka)); needed only when plane ka is inner
-    T Hc = ldp(Hin + psH * ka, oH, true);
-    const int kb = ka > 0 ? ka - 1 : 0;
+    T Hc = ldc<T>(bHin, oH + (uint32_t)(ka - kb) * pbH);
     Ta Dc{}, Dm1{}, Dm2{};  // aux at planes k, k-1, k-2
     if constexpr (AUX) {
-        Dc = ldp(D + psH * ka, oA, true);
-        Dm1 = ldp(D + psH * kb, oA, true);
+        Dc = ldc<Ta>(bD, oA + (uint32_t)(ka - kb) * pbA);
+        Dm1 = ldc<Ta>(bD, oA);
     }
     T unused = 0;
-    T q1zL = Ph::template face<false>(p, ldp(qzi + psQz * kb, oQz, true), ldp(Hin + psH * kb, oH, true), Hc,
-                                      Dm1, Dc, p.dz, p._dz, p._dz8, unused);
+    T q1zL = Ph::template face<false>(p, ldc<T>(bqz, oQz), ldc<T>(bHin, oH), Hc, Dm1, Dc, p.dz, p._dz, p._dz8,
+                                      unused);
 
     constexpr bool kAhead2 = t2_prefetch_depth<T>() == 2;
     PlaneOps2<T, Ta> A = load_plane(ka);
     PlaneOps2<T, Ta> B;
     if constexpr (kAhead2) B = load_plane(ka + 1 <= nz - 1 ? ka + 1 : nz - 1);
 
+    // correction sweep: level-1 damping factors of the faces / cell that level 2 evaluates next
+    constexpr bool LAG = DC == kDelta;
+    T w1x = 0, w1y = 0, w1zU = 0, v1 = 0;          // this iteration's level-1 factors (plane k)
+    T w1xp = 0, w1yp = 0, w1zL = 0, v1p = 0;       // carried: plane k-1, face (k-1 | k)
     T H1p = 0;             // H1 at plane k-1
     T Hop = 0;             // Hold at plane k-1
     T q1xp = 0, q1yp = 0;  // level-1 fluxes on the lower x / y faces at plane k-1
@@ -681,16 +712,29 @@ __device__ __forceinline__ void fused3d_t2_tile(const KP<T>& p, const T* __restr
             Dpn = sDp[rn][lane];
         }
         T q1x = 0, q1y = 0, q1zU = 0;
-        if (l1y) q1y = Ph::template face<false>(p, A.qy, Hs, Hc, Ds, Dc, p.dy, p._dy, p._dy8, unused);
-        if (l1all) {
-            q1x = Ph::template face<false>(p, A.qx, Hw, Hc, Dw, Dc, p.dx, p._dx, p._dx8, unused);
-            q1zU = Ph::template face<false>(p, A.qz, Hc, A.Hn, Dc, A.Dn, p.dz, p._dz, p._dz8, unused);
+        if constexpr (LAG) {
+            if (l1y) q1y = Ph::face_w(p, A.qy, Hs, Hc, Ds, Dc, w1y);
+            if (l1all) {
+                q1x = Ph::face_w(p, A.qx, Hw, Hc, Dw, Dc, w1x);
+                q1zU = Ph::face_w(p, A.qz, Hc, A.Hn, Dc, A.Dn, w1zU);
+            }
+        } else {
+            if (l1y) q1y = Ph::template face<false>(p, A.qy, Hs, Hc, Ds, Dc, p.dy, p._dy, p._dy8, unused);
+            if (l1all) {
+                q1x = Ph::template face<false>(p, A.qx, Hw, Hc, Dw, Dc, p.dx, p._dx, p._dx8, unused);
+                q1zU = Ph::template face<false>(p, A.qz, Hc, A.Hn, Dc, A.Dn, p.dz, p._dz, p._dz8, unused);
+            }
         }
         const T H1w = __shfl_up(H1p, 1);
         const T H1s = sH1[rs][lane];
         T r2x = 0, r2y = 0, q2x = 0, q2y = 0;
-        if (l2y) q2y = Ph::template face<CHECK>(p, q1yp, H1s, H1p, Dps, Dm1, p.dy, p._dy, p._dy8, r2y);
-        if (l2all) q2x = Ph::template face<CHECK>(p, q1xp, H1w, H1p, Dpw, Dm1, p.dx, p._dx, p._dx8, r2x);
+        if constexpr (LAG) {
+            if (l2y) q2y = Ph::template face_lag<CHECK>(q1yp, H1s, H1p, Dps, Dm1, w1yp, r2y);
+            if (l2all) q2x = Ph::template face_lag<CHECK>(q1xp, H1w, H1p, Dpw, Dm1, w1xp, r2x);
+        } else {
+            if (l2y) q2y = Ph::template face<CHECK>(p, q1yp, H1s, H1p, Dps, Dm1, p.dy, p._dy, p._dy8, r2y);
+            if (l2all) q2x = Ph::template face<CHECK>(p, q1xp, H1w, H1p, Dpw, Dm1, p.dx, p._dx, p._dx8, r2x);
+        }
         sQ1[row][lane] = q1y;
         sQ2[row][lane] = q2y;
         if constexpr (CHECK) sR2[row][lane] = r2y;
@@ -711,19 +755,27 @@ __device__ __forceinline__ void fused3d_t2_tile(const KP<T>& p, const T* __restr
         T H1 = Hc;  // boundary cells and rows without a level-1 update keep H
         if (l1all) {
             const T dv1 = Ph::template div<3>(p, q1xE - q1x, q1yN - q1y, q1zU - q1zL);
-            const T H1c = Ph::cell(p, Hc, A.Ho, dv1, dmax1, Dc);
+            T H1c;
+            if constexpr (LAG) H1c = Ph::cell_w(p, Hc, A.Ho, dv1, Dc, v1);
+            else H1c = Ph::cell(p, Hc, A.Ho, dv1, dmax1, Dc);
             if (u1 && k >= 1 && k <= nz - 2) H1 = H1c;
         }
 
         T r2z = 0, q2zU = 0;
-        if (l2all) q2zU = Ph::template face<CHECK>(p, q1zL, H1p, H1, Dm1, Dc, p.dz, p._dz, p._dz8, r2z);
+        if constexpr (LAG) {
+            if (l2all) q2zU = Ph::template face_lag<CHECK>(q1zL, H1p, H1, Dm1, Dc, w1zL, r2z);
+        } else {
+            if (l2all) q2zU = Ph::template face<CHECK>(p, q1zL, H1p, H1, Dm1, Dc, p.dz, p._dz, p._dz8, r2z);
+        }
         if (k > k0) {
             const int m = k - 1;
             if (l2all) {
                 const T q2xE = __shfl_down(q2x, 1);
                 const T q2yN = sQ2[rn][lane];
                 const T dv2 = Ph::template div<3>(p, q2xE - q2x, q2yN - q2y, q2zU - q2zL);
-                const T H2 = Ph::cell(p, H1p, Hop, dv2, dmax2, Dm1);
+                T H2;
+                if constexpr (LAG) H2 = Ph::cell_lag(p, H1p, Hop, dv2, v1p);
+                else H2 = Ph::cell(p, H1p, Hop, dv2, dmax2, Dm1);
                 if constexpr (CHECK) {
                     const T r2xE = __shfl_down(r2x, 1);
                     const T r2yN = sR2[rn][lane];
@@ -732,17 +784,24 @@ __device__ __forceinline__ void fused3d_t2_tile(const KP<T>& p, const T* __restr
                     acc += rr * rr;
                     amax = nanmax(fabs(rr), amax);
                 }
+                const uint32_t rm = (uint32_t)(m - kb);
                 if (u2) {
-                    stp(Hout + psH * m, oH, H2);
-                    stp(qzo + psQz * m, oQz, q2zU);
+                    stc<T>(bHout, oH + rm * pbH, H2);
+                    stc<T>(bqzo, oQz + rm * pbQz, q2zU);
                 }
-                if (w2x) stp(qxo + psQx * (m - 1), oQx, q2x);
+                if (w2x) stc<T>(bqxo, oQx + (rm - 1) * pbQx, q2x);
             }
-            if (w2y) stp(qyo + psQy * (m - 1), oQy, q2y);
+            if (w2y) stc<T>(bqyo, oQy + (uint32_t)(k - 2 - kb) * pbQy, q2y);
         } else if (k == k0 && k0 == 1) {
-            if (u2) stp(qzo, oQz, q2zU);  // face (0 | 1): lower face of the first inner plane
+            if (u2) stc<T>(bqzo, oQz, q2zU);  // face (0 | 1): lower face of the first inner plane (kb = 0)
         }
         // carry to plane k+1
+        if constexpr (LAG) {
+            w1xp = w1x;
+            w1yp = w1y;
+            w1zL = w1zU;
+            v1p = v1;
+        }
         q2zL = q2zU;
         r2zL = r2z;
         q1zL = q1zU;
@@ -856,11 +915,18 @@ static constexpr int kRows = 16;  // rows (waves) per block of the one-iteration
 // Planes per z chunk. Measured at 511^3 FP64 (FAST): 8 -> 2.20 ms, 16 -> 2.17, 32 -> 2.21, 64 -> 2.30,
 // 128 -> 2.45, 509 -> 3.01 ms per sweep: short chunks give many more blocks than CUs (small tail) and
 // the per-chunk prologue (one extra z face) stays cheap.
+// Deepest z chunk whose planes the fused 3D sweeps can address with 32-bit byte offsets from the
+// chunk's lowest plane: (tz + 5) planes of H under 4 GiB (0: none).
+int64_t max_chunk_tz(const ptd_params& p, int64_t esz) {
+    const int64_t plane = p.nx * p.ny * esz;
+    return std::max<int64_t>(0, ((int64_t)1 << 32) / std::max<int64_t>(plane, 1) - 5);
+}
+
 static int fused_tz(const ptd_ctx* c) {
     const char* env = getenv("PTD_TZ");
-    if (env && atoi(env) > 0) return atoi(env);
     const int64_t mz = c->p.nz - 2;
-    return (int)std::min<int64_t>(16, mz);
+    const int64_t want = env && atoi(env) > 0 ? atoi(env) : std::min<int64_t>(16, mz);
+    return (int)std::max<int64_t>(1, std::min(want, max_chunk_tz(c->p, (int64_t)c->esz)));
 }
 
 // Buffers of one sweep: iterate n in, iterate n+1 (or n+2) out.
@@ -1038,7 +1104,7 @@ hipError_t launch_iter_params(ptd_ctx* c) {
 }
 
 bool fused_available(const ptd_ctx* c) {
-    return (c->p.ndim == 3 && c->p.nx >= 3 && c->p.ny >= 3 && c->p.nz >= 3) ||
+    return (c->p.ndim == 3 && c->p.nx >= 3 && c->p.ny >= 3 && c->p.nz >= 3 && max_chunk_tz(c->p, (int64_t)c->esz) >= 1) ||
            (c->p.ndim == 2 && c->p.nx >= 3 && c->p.ny >= 3);
 }
 bool fused2_available(const ptd_ctx* c) { return c->p.ndim == 3 && fused_available(c); }
@@ -1059,12 +1125,13 @@ hipError_t launch_f64_to_f32(ptd_ctx* c, float* dst, const double* src, int64_t 
 }
 
 // Face state conversion between the true fluxes q (float64, they persist over physical steps as the
-// reference's qHx/qHy/qHz do) and the correction fluxes p = q + g(Hc) (float32) of the correction
-// sweep, g(Hc) = coef * dHc / h the equilibrium flux magnitude of the float64 iterate.
+// reference's qHx/qHy/qHz do) and the correction fluxes p = q + g(Hc) (float32, stored times sc, the
+// axis' scale of the correction sweep) of the correction sweep, g(Hc) = coef * dHc / h the equilibrium
+// flux magnitude of the float64 iterate.
 template <int DC, int AXIS, bool TO_P>
 __global__ void __launch_bounds__(kBlock)
 k_flux_state(const KP<double> p, const double* __restrict__ H, const double* __restrict__ D,
-             double* __restrict__ q, float* __restrict__ pf, int64_t count) {
+             double* __restrict__ q, float* __restrict__ pf, int64_t count, double sc, double isc) {
     const int64_t t = (int64_t)blockIdx.x * kBlock + threadIdx.x;
     if (t >= count) return;
     int64_t a, step;
@@ -1096,8 +1163,8 @@ k_flux_state(const KP<double> p, const double* __restrict__ H, const double* __r
         coef = p.d0;
     }
     const double g = coef * (ih * (Hb - Ha));
-    if constexpr (TO_P) pf[t] = (float)(q[t] + g);
-    else q[t] = (double)pf[t] - g;
+    if constexpr (TO_P) pf[t] = (float)((q[t] + g) * sc);
+    else q[t] = (double)pf[t] * isc - g;
 }
 
 template <int DC, bool TO_P>
@@ -1111,9 +1178,13 @@ static hipError_t flux_state_t(ptd_ctx* c) {
         double* q = (double*)c->q[a][c->qcur];
         float* pf = (float*)c->pq[a][c->pcur];
         const dim3 g(grid_for(n)), bl(kBlock);
-        if (a == 0) hipLaunchKernelGGL((k_flux_state<DC, 0, TO_P>), g, bl, 0, c->stream, kp, H, D, q, pf, n);
-        else if (a == 1) hipLaunchKernelGGL((k_flux_state<DC, 1, TO_P>), g, bl, 0, c->stream, kp, H, D, q, pf, n);
-        else hipLaunchKernelGGL((k_flux_state<DC, 2, TO_P>), g, bl, 0, c->stream, kp, H, D, q, pf, n);
+        const double sc = corr_flux_scale(c->p, DC, a), isc = 1.0 / sc;
+        if (a == 0)
+            hipLaunchKernelGGL((k_flux_state<DC, 0, TO_P>), g, bl, 0, c->stream, kp, H, D, q, pf, n, sc, isc);
+        else if (a == 1)
+            hipLaunchKernelGGL((k_flux_state<DC, 1, TO_P>), g, bl, 0, c->stream, kp, H, D, q, pf, n, sc, isc);
+        else
+            hipLaunchKernelGGL((k_flux_state<DC, 2, TO_P>), g, bl, 0, c->stream, kp, H, D, q, pf, n, sc, isc);
     }
     return hipGetLastError();
 }
@@ -1166,14 +1237,15 @@ k_outer_residual(const KP<double> p, const double* __restrict__ H, const double*
 }
 
 // End of a correction (one pass over every cell): the correction fluxes move from Hc to Hc + delta,
-// p <- float((double(p) - g(Hc)) + g(Hc + delta)) on each face (the face whose upper cell is this cell;
-// g = coef * dH / h the equilibrium flux magnitude, as in k_flux_state), and Hc + delta goes to the
-// other H buffer (inner cells; the boundary cells hold the same values in both buffers).
+// p <- float((double(p)/sc - g(Hc) + g(Hc + delta))*sc) on each face (the face whose upper cell is this
+// cell; g = coef * dH / h the equilibrium flux magnitude, as in k_flux_state; sc the axis' stored scale),
+// and Hc + delta goes to the other H buffer (inner cells; the boundary cells hold the same values in
+// both buffers).
 template <int DC>
 __global__ void __launch_bounds__(kBlock)
 k_outer_update(const KP<double> p, const double* __restrict__ Hin, double* __restrict__ Hout,
                const float* __restrict__ dl, const double* __restrict__ D, float* __restrict__ px,
-               float* __restrict__ py, float* __restrict__ pz, int64_t n) {
+               float* __restrict__ py, float* __restrict__ pz, int64_t n, double3 sc, double3 isc) {
     const int64_t g = (int64_t)blockIdx.x * kBlock + threadIdx.x;
     if (g >= n) return;
     const int64_t nx = p.nx, ny = p.ny, nz = p.nz, pl = nx * ny;
@@ -1191,15 +1263,18 @@ k_outer_update(const KP<double> p, const double* __restrict__ Hin, double* __res
         }
         return coef * (ih * (Hb - Ha));
     };
-    auto move = [&](float* pf, int64_t f, int64_t a, double ih) {
+    auto move = [&](float* pf, int64_t f, int64_t a, double ih, double s, double is) {
         const double Ha = Hin[a], Hb = Hin[g];
         const double go = gflux(Ha, Hb, a, g, ih);
         const double gn = gflux(Ha + (double)dl[a], Hb + (double)dl[g], a, g, ih);
-        pf[f] = (float)(((double)pf[f] - go) + gn);
+        pf[f] = (float)(((double)pf[f] * is - go + gn) * s);
     };
-    if (i >= 1 && ji && ki) move(px, (i - 1) + (nx - 1) * ((j - 1) + (ny - 2) * (k - 1)), g - 1, p._dx);
-    if (j >= 1 && ii && ki) move(py, (i - 1) + (nx - 2) * ((j - 1) + (ny - 1) * (k - 1)), g - nx, p._dy);
-    if (k >= 1 && ii && ji) move(pz, (i - 1) + (nx - 2) * ((j - 1) + (ny - 2) * (k - 1)), g - pl, p._dz);
+    if (i >= 1 && ji && ki)
+        move(px, (i - 1) + (nx - 1) * ((j - 1) + (ny - 2) * (k - 1)), g - 1, p._dx, sc.x, isc.x);
+    if (j >= 1 && ii && ki)
+        move(py, (i - 1) + (nx - 2) * ((j - 1) + (ny - 1) * (k - 1)), g - nx, p._dy, sc.y, isc.y);
+    if (k >= 1 && ii && ji)
+        move(pz, (i - 1) + (nx - 2) * ((j - 1) + (ny - 2) * (k - 1)), g - pl, p._dz, sc.z, isc.z);
     if (ii && ji && ki) Hout[g] = Hin[g] + (double)dl[g];
 }
 
@@ -1235,12 +1310,19 @@ hipError_t launch_outer_update(ptd_ctx* c) {
     float* px = (float*)c->pq[0][c->pcur];
     float* py = (float*)c->pq[1][c->pcur];
     float* pz = (float*)c->pq[2][c->pcur];
-    if (c->p.dcase == PTD_NONLIN)
-        hipLaunchKernelGGL((k_outer_update<PTD_NONLIN>), g, bl, 0, c->stream, kp, Hin, Hout, dl, D, px, py, pz, c->nH);
-    else if (c->p.dcase == PTD_STEP)
-        hipLaunchKernelGGL((k_outer_update<PTD_STEP>), g, bl, 0, c->stream, kp, Hin, Hout, dl, D, px, py, pz, c->nH);
+    const int dc = c->p.dcase;
+    const double3 sc = make_double3(corr_flux_scale(c->p, dc, 0), corr_flux_scale(c->p, dc, 1),
+                                    corr_flux_scale(c->p, dc, 2));
+    const double3 isc = make_double3(1.0 / sc.x, 1.0 / sc.y, 1.0 / sc.z);
+    if (dc == PTD_NONLIN)
+        hipLaunchKernelGGL((k_outer_update<PTD_NONLIN>), g, bl, 0, c->stream, kp, Hin, Hout, dl, D, px, py, pz, c->nH,
+                           sc, isc);
+    else if (dc == PTD_STEP)
+        hipLaunchKernelGGL((k_outer_update<PTD_STEP>), g, bl, 0, c->stream, kp, Hin, Hout, dl, D, px, py, pz, c->nH,
+                           sc, isc);
     else
-        hipLaunchKernelGGL((k_outer_update<PTD_LIN>), g, bl, 0, c->stream, kp, Hin, Hout, dl, D, px, py, pz, c->nH);
+        hipLaunchKernelGGL((k_outer_update<PTD_LIN>), g, bl, 0, c->stream, kp, Hin, Hout, dl, D, px, py, pz, c->nH,
+                           sc, isc);
     hipError_t e = hipGetLastError();
     if (e == hipSuccess) c->hcur = 1 - c->hcur;
     return e;
