@@ -261,14 +261,19 @@ int run_delta(ptd_ctx* c, int64_t n, bool check_last) {
 // corrections continue one PT iteration. An inner solve stops once it has reduced its residual
 // kInner-fold, or reached kLast*tol (the last correction), or stalls. The float32 floor of one
 // correction solve (rounding of delta, amplified by D/dx^2) lies ~3.5 decades below its starting
-// residual, and a restart from floor noise converges markedly slower than the float64 iteration; two
-// decades per correction keep every restart clear of it. Checks are placed by prediction: the first
-// after 70% of the sweeps the last complete correction took (scaled by the decades to go), the next
-// ones after the sweeps the observed decay rate predicts. Measured at 511^3 (5 steps): 3940-3970
-// float32 sweeps and ~70-95 checks, vs 3860 float64 iterations checked every 10.
+// residual at 511^3 and closer at larger n, and sweeps near it converge markedly slower than the
+// float64 iteration; kInner keeps every correction clear of it. Checks are placed by prediction: the
+// first after 70% of the sweeps the last complete correction took (scaled by the decades to go), the
+// next ones after the sweeps the observed decay rate predicts. Measured (5 steps): 3910 float32 sweeps
+// at 511^3 vs 3860 float64 iterations checked every 10; 7830 at 1023^3 vs 7670.
 int pt_solve_mixed(ptd_ctx* c, int64_t* iters, double* err_rms, double* err_max) {
-    constexpr double kInner = 1e-2, kLast = 0.85;
     const ptd_params& p = c->p;
+    // reduction asked of one correction solve: its float32 floor lies a fixed factor times the
+    // condition number (~ n^2) below its start, so the reduction per correction shrinks as n grows
+    // (measured optimum: 1e-2 .. 3e-2 at 511^3, 5e-2 .. 1e-1 at 1023^3)
+    const double nmax = (double)std::max(p.nx, std::max(p.ny, p.nz));
+    const double kInner = std::min(1e-1, std::max(1e-2, 2.5e-2 * (nmax / 511.0) * (nmax / 511.0)));
+    constexpr double kLast = 0.85;
     if (p.dcase == PTD_STEP) PTD_HIP(ptd::launch_f64_to_f32(c, (float*)c->Df, (const double*)c->D, c->nH));
     // residual of the float64 system at Hc, with the next correction's source, float32(Hc), delta = 0
     auto outer_residual = [&](double* rms, double* mx) -> int {
