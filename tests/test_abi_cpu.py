@@ -57,7 +57,8 @@ def test_null_context_is_an_argument_error(pkg):
     assert b"NULL" in lib.ptd_last_error()
 
 
-@pytest.mark.parametrize("what", ["1d", "fp32", "perf", "odd_nout", "itmax", "nout_mismatch", "empty"])
+@pytest.mark.parametrize("what", ["1d", "fp32", "perf", "odd_nout", "itmax", "nout_mismatch", "empty",
+                                  "big_plane"])
 def test_batch_rejects_invalid_members_before_device(what, pkg):
     from importlib import import_module
     abi = import_module("ptd_amd._abi")
@@ -79,6 +80,27 @@ def test_batch_rejects_invalid_members_before_device(what, pkg):
         members[1].nout = 20
     elif what == "empty":
         members = []
+    elif what == "big_plane":
+        # 37 x-y planes of 4000^2 float64 exceed the 4 GiB span of the batched sweep's 32-bit offsets
+        members[1] = pkg.make_params(pkg.SCRIPTS["diff_3D_nonlin_multixpu"], 4000, 4000, 16)
     with pytest.raises(pkg.PTDError) as e:
         pkg.Batch(members)
     assert e.value.status == abi.ERR_ARG
+    if what == "big_plane":
+        assert b"plane too large" in pkg.load().ptd_last_error()
+
+
+@pytest.mark.parametrize("nxy,ok", [(511, True), (8000, True), (9500, False), (30000, False)])
+def test_mixed_rejects_planes_beyond_the_sweep_offsets(nxy, ok, pkg):
+    """PTD_MIXED always runs the fused 3D correction sweeps, which address (tz + 5) x-y planes from a chunk base
+    with 32-bit byte offsets: a float64 plane of nx*ny cells must satisfy 6 * nx*ny * 8 < 2^32."""
+    from importlib import import_module
+    abi = import_module("ptd_amd._abi")
+    assert ok == (6 * nxy * nxy * 8 < 2 ** 32)
+    p = pkg.make_params(pkg.SCRIPTS["diff_3D_nonlin_multixpu"], nxy, nxy, 4, precision=abi.MIXED)
+    if ok:
+        return  # creating it would allocate; the rule above is what the library applies
+    with pytest.raises(pkg.PTDError) as e:
+        pkg.Context(p)
+    assert e.value.status == abi.ERR_ARG
+    assert b"4 GiB" in pkg.load().ptd_last_error()
