@@ -1476,6 +1476,14 @@ hipError_t launch_grad_accum(ptd_ctx* c, const double* mu, const double* Hn, dou
 // function as the single-problem sweep on that member's fields. Buffers come from the member's
 // parities + the launch index li in the check block; members that are done exit at once.
 
+// A pointer read from device memory (the member table) is a generic pointer to the compiler, which
+// then emits flat accesses with 64-bit vector addresses; re-deriving it in the global address space
+// gives global accesses with scalar bases. Every member array is device memory from hipMalloc.
+template <typename T>
+__device__ __forceinline__ T* as_global(T* ptr) {
+    return (T*)(__attribute__((address_space(1))) T*)(uintptr_t)ptr;
+}
+
 template <int DC, int ARITH, int R, bool CHECK>
 __global__ void __launch_bounds__(64 * R, (t2_waves_per_simd<double, DC, ARITH, R, CHECK>()))
 k_batch3d_t2(BatchMember* __restrict__ mem, const BatchTile* __restrict__ tiles, int ntot, int per_xcd, int li) {
@@ -1493,8 +1501,9 @@ k_batch3d_t2(BatchMember* __restrict__ mem, const BatchTile* __restrict__ tiles,
     const int hp = (M.hpar + li) & 1, qp = (M.qpar + li) & 1;
     const int rest = tile / ntx;
     fused3d_t2_tile<double, DC, PTD_PLAIN, ARITH, R, CHECK>(
-        M.p, M.H[hp], M.H[hp ^ 1], M.Hold, M.D, M.q[0][qp], M.q[1][qp], M.q[2][qp], M.q[0][qp ^ 1],
-        M.q[1][qp ^ 1], M.q[2][qp ^ 1], M.partial + tile, tile % ntx, rest % nty, rest / nty, tz);
+        M.p, as_global(M.H[hp]), as_global(M.H[hp ^ 1]), as_global(M.Hold), as_global(M.D), as_global(M.q[0][qp]),
+        as_global(M.q[1][qp]), as_global(M.q[2][qp]), as_global(M.q[0][qp ^ 1]), as_global(M.q[1][qp ^ 1]),
+        as_global(M.q[2][qp ^ 1]), as_global(M.partial + tile), tile % ntx, rest % nty, rest / nty, tz);
 }
 
 template <int DC, int ARITH, int R, bool CHECK>
@@ -1510,9 +1519,10 @@ k_batch2d(BatchMember* __restrict__ mem, const BatchTile* __restrict__ tiles, in
     if (!M.active) return;
     const int ntx = ((int)M.p.nx - 2 + 61) / 62;
     const int hp = (M.hpar + li) & 1, qp = (M.qpar + li) & 1;
-    fused2d_tile<double, DC, PTD_PLAIN, ARITH, R, CHECK>(M.p, M.H[hp], M.H[hp ^ 1], M.Hold, M.D, M.q[0][qp],
-                                                         M.q[1][qp], M.q[0][qp ^ 1], M.q[1][qp ^ 1],
-                                                         M.partial + tile, tile % ntx, tile / ntx);
+    fused2d_tile<double, DC, PTD_PLAIN, ARITH, R, CHECK>(
+        M.p, as_global(M.H[hp]), as_global(M.H[hp ^ 1]), as_global(M.Hold), as_global(M.D), as_global(M.q[0][qp]),
+        as_global(M.q[1][qp]), as_global(M.q[0][qp ^ 1]), as_global(M.q[1][qp ^ 1]), as_global(M.partial + tile),
+        tile % ntx, tile / ntx);
 }
 
 // rows of the batched launches: the checking launch uses the single-problem checking tiling (so the
