@@ -156,7 +156,7 @@ int enqueue_block(ptd_batch* b, int cand) {
         const int lpb = g.ndim == 3 ? b->nout / 2 : b->nout;
         for (int li = 0; li < lpb; ++li) {
             const bool check = li == lpb - 1;
-            const int rows = check ? ptd::batch_rows_check(g.ndim)
+            const int rows = check ? ptd::batch_rows_check(g.ndim, g.dcase, g.arith)
                                    : ptd::batch_rows_nocheck(g.ndim, g.dcase, g.arith, kCandidates[cand].rows2d);
             PTD_HIPB(ptd::launch_batch_sweep(g.stream, g.ndim, g.dcase, g.arith, rows, check, b->dm,
                                              check ? g.d_check : g.d_nc[cand], check ? g.n_check : g.n_nc[cand], li));
@@ -273,7 +273,7 @@ int32_t ptd_batch_create(const ptd_params* params, int32_t n, int32_t device, pt
         if (st == PTD_OK && p.dcase == PTD_STEP) st = balloc(b, (void**)&M.D, (size_t)M.nH * 8);
         for (int a = 0; a < 3 && st == PTD_OK; ++a)
             for (int k = 0; k < 2 && st == PTD_OK; ++k) st = balloc(b, (void**)&M.q[a][k], (size_t)nq[a] * 8);
-        M.ntiles_check = member_tiles(p, ptd::batch_rows_check((int)p.ndim), check_tz(p));
+        M.ntiles_check = member_tiles(p, ptd::batch_rows_check((int)p.ndim, p.dcase, p.arith), check_tz(p));
         if (st == PTD_OK) st = balloc(b, (void**)&M.partial, (size_t)M.ntiles_check * sizeof(double2));
         if (st != PTD_OK) return fail_free(st);
         M.itmax = p.itmax;
@@ -316,7 +316,7 @@ int32_t ptd_batch_create(const ptd_params* params, int32_t n, int32_t device, pt
         for (int32_t m : g.members) {
             const ptd_params& p = params[m];
             const int tz = p.ndim == 3 ? check_tz(p) : 0;
-            const int nt = member_tiles(p, ptd::batch_rows_check(g.ndim), tz);
+            const int nt = member_tiles(p, ptd::batch_rows_check(g.ndim, g.dcase, g.arith), tz);
             for (int k = 0; k < nt; ++k) t.push_back({m, k, tz, 0});
         }
         g.n_check = (int)t.size();

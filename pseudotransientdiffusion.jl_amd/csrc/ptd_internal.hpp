@@ -544,7 +544,7 @@ int set_error(int code, const std::string& msg);   // sets ptd_last_error(), ret
 hipError_t launch_batch_sweep(hipStream_t s, int ndim, int dcase, int arith, int rows, bool check,
                               BatchMember* mem, const BatchTile* tiles, int ntiles, int li);
 int batch_rows_nocheck(int ndim, int dcase, int arith, int variant);  // rows of the non-checking launch
-int batch_rows_check(int ndim);                                       // rows of the checking launch
+int batch_rows_check(int ndim, int dcase, int arith);                                       // rows of the checking launch
 hipError_t launch_batch_reduce(hipStream_t s, BatchMember* mem, const int32_t* members, int n);
 hipError_t launch_batch_control(hipStream_t s, BatchMember* mem, int n, int64_t* iters, double* errs,
                                 int32_t* active_count, int nt_cap);

@@ -545,15 +545,15 @@ struct PlaneOps2 {
 // values (checking, step D, EXACT) would spill under that cap and run 16 rows, one block per CU.
 template <typename T, int DC, int ARITH, bool CHECK>
 constexpr bool t2_compact() {
-    return sizeof(T) == 8 && ARITH == kFast && DC != PTD_STEP && !CHECK;
+    return sizeof(T) == 8 && ARITH == kFast && DC != PTD_STEP;
 }
 template <typename T, int DC, int ARITH, int R, bool CHECK>
 constexpr int t2_waves_per_simd() {
     return sizeof(T) == 4 ? 2 * (R / 4) : (t2_compact<T, DC, ARITH, CHECK>() ? 2 * (R / 4) : 1);
 }
-template <typename T>
+template <typename T, bool CHECK>
 constexpr int t2_prefetch_depth() {
-    return sizeof(T) == 4 ? 1 : 2;
+    return sizeof(T) == 4 || CHECK ? 1 : 2;
 }
 template <typename T, int DC, int ARITH, bool CHECK>
 constexpr int t2_rows() {
@@ -671,7 +671,7 @@ __device__ __forceinline__ void fused3d_t2_tile(const KP<T>& p, const T* __restr
     T q1zL = Ph::template face<false>(p, ldc<T>(bqz, oQz), ldc<T>(bHin, oH), Hc, Dm1, Dc, p.dz, p._dz, p._dz8,
                                       unused);
 
-    constexpr bool kAhead2 = t2_prefetch_depth<T>() == 2;
+    constexpr bool kAhead2 = t2_prefetch_depth<T, CHECK>() == 2;
     PlaneOps2<T, Ta> A = load_plane(ka);
     PlaneOps2<T, Ta> B;
     if constexpr (kAhead2) B = load_plane(ka + 1 <= nz - 1 ? ka + 1 : nz - 1);
@@ -1528,7 +1528,11 @@ k_batch2d(BatchMember* __restrict__ mem, const BatchTile* __restrict__ tiles, in
 // rows of the batched launches: the checking launch uses the single-problem checking tiling (so the
 // residual partials, and hence every stopping decision, are those of a single-problem solve); the
 // other launches may use any tiling (every cell is computed by its owner with the same expressions).
-int batch_rows_check(int ndim) { return ndim == 3 ? 16 : kRows2D; }
+int batch_rows_check(int ndim, int dcase, int arith) {
+    if (ndim != 3) return kRows2D;
+    if (arith != PTD_ARITH_FAST) return t2_rows<double, PTD_NONLIN, kExact, true>();
+    return dcase == PTD_STEP ? t2_rows<double, PTD_STEP, kFast, true>() : t2_rows<double, PTD_NONLIN, kFast, true>();
+}
 int batch_rows_nocheck(int ndim, int dcase, int arith, int variant) {
     if (ndim == 2) return variant == 1 ? 8 : 16;
     return (arith == PTD_ARITH_FAST && dcase != PTD_STEP) ? 12 : 16;
@@ -1540,10 +1544,10 @@ static hipError_t batch_sweep_t(hipStream_t s, int ndim, int rows, bool check, B
     const int per = (ntiles + 7) / 8;
     const dim3 g(8 * per);
     if (ndim == 3) {
-        constexpr int RN = (ARITH == kFast && DC != PTD_STEP) ? 12 : 16;
+        constexpr int RN = t2_rows<double, DC, ARITH, false>(), RC = t2_rows<double, DC, ARITH, true>();
         if (check) {
-            if (rows != 16) return hipErrorInvalidValue;
-            hipLaunchKernelGGL((k_batch3d_t2<DC, ARITH, 16, true>), g, dim3(64, 16), 0, s, mem, tiles, ntiles, per, li);
+            if (rows != RC) return hipErrorInvalidValue;
+            hipLaunchKernelGGL((k_batch3d_t2<DC, ARITH, RC, true>), g, dim3(64, RC), 0, s, mem, tiles, ntiles, per, li);
         } else {
             if (rows != RN) return hipErrorInvalidValue;
             hipLaunchKernelGGL((k_batch3d_t2<DC, ARITH, RN, false>), g, dim3(64, RN), 0, s, mem, tiles, ntiles, per, li);
