@@ -348,9 +348,10 @@ int pt_solve_mixed(ptd_ctx* c, int64_t* iters, double* err_rms, double* err_max)
         }
         total += it;
         if (met) c->inner_hint = it;
-        // p moves to Hc + delta and Hc += delta (one pass), then the next correction's residual
-        PTD_HIP(ptd::launch_outer_update(c));
-        st = outer_residual(&rms, &mx);
+        // one pass: p moves to Hc + delta, Hc += delta, and the next correction's residual, source,
+        // float32(Hc) and delta = 0
+        PTD_HIP(ptd::launch_outer_step(c));
+        st = fetch_errors(c, &rms, &mx);
         if (st != PTD_OK) return st;
         err = p.stop_norm == PTD_NORM_MAX ? mx : rms;
         ++outer;

@@ -492,10 +492,12 @@ hipError_t launch_f64_to_f32(ptd_ctx* c, float* dst, const double* src, int64_t 
 // correction fluxes <-> true fluxes: to_p: p = float(q + g(H)); !to_p: q = double(p) - g(H), where g(H)
 // is the equilibrium flux magnitude coef*dH/h of the float64 iterate H on each face
 hipError_t launch_flux_state(ptd_ctx* c, bool to_p);
-// fused correction boundaries: residual + source + float32(Hc) + delta = 0 (partials -> dred), and
-// p <- p + g(Hc + delta) - g(Hc) with Hc + delta into the other H buffer (flips hcur)
+// correction boundaries. outer_residual (start of a PT solve): residual of the float64 system at Hc +
+// source + float32(Hc) + delta = 0 (partials -> dred). outer_step (after a correction, one pass):
+// p <- p + g(Hc + delta) - g(Hc), Hc + delta into the other H buffer (flips hcur), then the same outputs
+// at the new Hc, with delta = 0 in the other delta buffer (flips dcur)
 hipError_t launch_outer_residual(ptd_ctx* c);
-hipError_t launch_outer_update(ptd_ctx* c);
+hipError_t launch_outer_step(ptd_ctx* c);
 
 // gradients of the lin / step solves (ptd_grad.hip): adjoint source (H - Hobs)*dt with L's partials
 // reduced into c->dred, and one step's gradient contribution

@@ -1236,46 +1236,106 @@ k_outer_residual(const KP<double> p, const double* __restrict__ H, const double*
     block_sum_max(s, m, threadIdx.x, kBlock, &partial[blockIdx.x]);
 }
 
-// End of a correction (one pass over every cell): the correction fluxes move from Hc to Hc + delta,
-// p <- float((double(p)/sc - g(Hc) + g(Hc + delta))*sc) on each face (the face whose upper cell is this
-// cell; g = coef * dH / h the equilibrium flux magnitude, as in k_flux_state; sc the axis' stored scale),
-// and Hc + delta goes to the other H buffer (inner cells; the boundary cells hold the same values in
-// both buffers).
-template <int DC>
+// Correction boundary in one pass (flux-state move, Hc += delta and k_outer_residual's outputs): with the new
+// iterate Hn = Hc + delta evaluated on the fly at every cell and its 6 neighbours (delta is 0 on the
+// boundary cells), the correction fluxes move from Hc to Hn, Hn goes to the other H buffer, and the
+// residual of the float64 system at Hn gives ResH, the next source R*dt, float32(Hn) and the partials.
+// The next correction's delta = 0 goes to the other delta buffer, so no thread writes what another reads.
+template <int DC, int FORM>
 __global__ void __launch_bounds__(kBlock)
-k_outer_update(const KP<double> p, const double* __restrict__ Hin, double* __restrict__ Hout,
-               const float* __restrict__ dl, const double* __restrict__ D, float* __restrict__ px,
-               float* __restrict__ py, float* __restrict__ pz, int64_t n, double3 sc, double3 isc) {
+k_outer_step(const KP<double> p, const double* __restrict__ Hin, double* __restrict__ Hout,
+             const float* __restrict__ dl, float* __restrict__ dl_next, const double* __restrict__ Hold,
+             const double* __restrict__ D, double* __restrict__ resh, float* __restrict__ src,
+             float* __restrict__ hcf, float* __restrict__ px, float* __restrict__ py, float* __restrict__ pz,
+             double3 sc, double3 isc, double2* __restrict__ partial, int64_t n) {
+    using Ph = Phys<double, DC, FORM, kExact>;
     const int64_t g = (int64_t)blockIdx.x * kBlock + threadIdx.x;
-    if (g >= n) return;
-    const int64_t nx = p.nx, ny = p.ny, nz = p.nz, pl = nx * ny;
-    const int64_t i = g % nx, rest = g / nx, j = rest % ny, k = rest / ny;
-    const bool ii = i >= 1 && i <= nx - 2, ji = j >= 1 && j <= ny - 2, ki = k >= 1 && k <= nz - 2;
-    auto gflux = [&](double Ha, double Hb, int64_t a, int64_t b, double ih) {
-        double coef;
-        if constexpr (DC == PTD_NONLIN) {
-            const double hf = 0.5 * (Ha + Hb);
-            coef = hf * hf * hf;
-        } else if constexpr (DC == PTD_STEP) {
-            coef = (D[a] + D[b]) * 0.5;
+    double s = 0.0, m = 0.0;
+    if (g < n) {
+        const int64_t nx = p.nx, ny = p.ny, nz = p.nz, pl = nx * ny;
+        const int64_t i = g % nx, rest = g / nx, j = rest % ny, k = rest / ny;
+        const bool ii = i >= 1 && i <= nx - 2, ji = j >= 1 && j <= ny - 2, ki = k >= 1 && k <= nz - 2;
+        auto Hn = [&](int64_t o) { return Hin[o] + (double)dl[o]; };
+        auto Dv = [&](int64_t o) { return DC == PTD_STEP ? D[o] : 0.0; };
+        auto gflux = [&](double Ha, double Hb, int64_t a, int64_t b, double ih) {
+            double coef;
+            if constexpr (DC == PTD_NONLIN) {
+                const double hf = 0.5 * (Ha + Hb);
+                coef = hf * hf * hf;
+            } else if constexpr (DC == PTD_STEP) {
+                coef = (D[a] + D[b]) * 0.5;
+            } else {
+                coef = p.d0;
+            }
+            return coef * (ih * (Hb - Ha));
+        };
+        auto move = [&](float* pf, int64_t f, int64_t a, double ih, double sa, double isa) {
+            const double go = gflux(Hin[a], Hin[g], a, g, ih);
+            const double gn = gflux(Hn(a), Hn(g), a, g, ih);
+            pf[f] = (float)(((double)pf[f] * isa - go + gn) * sa);
+        };
+        if (i >= 1 && ji && ki)
+            move(px, (i - 1) + (nx - 1) * ((j - 1) + (ny - 2) * (k - 1)), g - 1, p._dx, sc.x, isc.x);
+        if (j >= 1 && ii && ki)
+            move(py, (i - 1) + (nx - 2) * ((j - 1) + (ny - 1) * (k - 1)), g - nx, p._dy, sc.y, isc.y);
+        if (k >= 1 && ii && ji)
+            move(pz, (i - 1) + (nx - 2) * ((j - 1) + (ny - 2) * (k - 1)), g - pl, p._dz, sc.z, isc.z);
+        const double Hc = Hn(g);
+        if (hcf) hcf[g] = (float)Hc;
+        dl_next[g] = 0.0f;
+        if (ii && ji && ki) {
+            Hout[g] = Hc;
+            const double Dc = Dv(g);
+            const double fxw = Ph::face_res(p, Hn(g - 1), Hc, Dv(g - 1), Dc, p.dx, p._dx);
+            const double fxe = Ph::face_res(p, Hc, Hn(g + 1), Dc, Dv(g + 1), p.dx, p._dx);
+            const double fys = Ph::face_res(p, Hn(g - nx), Hc, Dv(g - nx), Dc, p.dy, p._dy);
+            const double fyn = Ph::face_res(p, Hc, Hn(g + nx), Dc, Dv(g + nx), p.dy, p._dy);
+            const double fzd = Ph::face_res(p, Hn(g - pl), Hc, Dv(g - pl), Dc, p.dz, p._dz);
+            const double fzu = Ph::face_res(p, Hc, Hn(g + pl), Dc, Dv(g + pl), p.dz, p._dz);
+            const double dv = divergence<double, FORM, 3>(p, fxe - fxw, fyn - fys, fzu - fzd);
+            const double r = residual<double, FORM>(p, Hc, Hold[g], dv);
+            resh[(i - 1) + p.mx * ((j - 1) + p.my * (k - 1))] = r;
+            src[g] = (float)(r * p.dt);
+            s = r * r;
+            m = fabs(r);
         } else {
-            coef = p.d0;
+            src[g] = 0.0f;
         }
-        return coef * (ih * (Hb - Ha));
-    };
-    auto move = [&](float* pf, int64_t f, int64_t a, double ih, double s, double is) {
-        const double Ha = Hin[a], Hb = Hin[g];
-        const double go = gflux(Ha, Hb, a, g, ih);
-        const double gn = gflux(Ha + (double)dl[a], Hb + (double)dl[g], a, g, ih);
-        pf[f] = (float)(((double)pf[f] * is - go + gn) * s);
-    };
-    if (i >= 1 && ji && ki)
-        move(px, (i - 1) + (nx - 1) * ((j - 1) + (ny - 2) * (k - 1)), g - 1, p._dx, sc.x, isc.x);
-    if (j >= 1 && ii && ki)
-        move(py, (i - 1) + (nx - 2) * ((j - 1) + (ny - 1) * (k - 1)), g - nx, p._dy, sc.y, isc.y);
-    if (k >= 1 && ii && ji)
-        move(pz, (i - 1) + (nx - 2) * ((j - 1) + (ny - 2) * (k - 1)), g - pl, p._dz, sc.z, isc.z);
-    if (ii && ji && ki) Hout[g] = Hin[g] + (double)dl[g];
+    }
+    block_sum_max(s, m, threadIdx.x, kBlock, &partial[blockIdx.x]);
+}
+
+template <int DC, int FORM>
+static hipError_t outer_step_t(ptd_ctx* c) {
+    const KP<double> kp = make_kp<double>(c->p);
+    const unsigned gb = grid_for(c->nH);
+    if ((int64_t)gb > c->npartial) return hipErrorInvalidValue;
+    const int dc = c->p.dcase;
+    const double3 sc = make_double3(corr_flux_scale(c->p, dc, 0), corr_flux_scale(c->p, dc, 1),
+                                    corr_flux_scale(c->p, dc, 2));
+    const double3 isc = make_double3(1.0 / sc.x, 1.0 / sc.y, 1.0 / sc.z);
+    hipLaunchKernelGGL((k_outer_step<DC, FORM>), dim3(gb), dim3(kBlock), 0, c->stream, kp,
+                       (const double*)c->H[c->hcur], (double*)c->H[1 - c->hcur], (const float*)c->dl[c->dcur],
+                       (float*)c->dl[1 - c->dcur], (const double*)c->Hold, (const double*)c->D, (double*)c->resh,
+                       (float*)c->src, (float*)c->hcf, (float*)c->pq[0][c->pcur], (float*)c->pq[1][c->pcur],
+                       (float*)c->pq[2][c->pcur], sc, isc, (double2*)c->partial, c->nH);
+    hipLaunchKernelGGL(k_reduce_partials, dim3(1), dim3(kBlock), 0, c->stream, (const double2*)c->partial,
+                       (int64_t)gb, c->dred);
+    hipError_t e = hipGetLastError();
+    if (e == hipSuccess) {
+        c->hcur = 1 - c->hcur;
+        c->dcur = 1 - c->dcur;
+    }
+    return e;
+}
+
+hipError_t launch_outer_step(ptd_ctx* c) {
+    const bool perf = c->p.form == PTD_PERF;
+    switch (c->p.dcase) {
+        case PTD_LIN: return perf ? outer_step_t<PTD_LIN, PTD_PERF>(c) : outer_step_t<PTD_LIN, PTD_PLAIN>(c);
+        case PTD_STEP: return perf ? outer_step_t<PTD_STEP, PTD_PERF>(c) : outer_step_t<PTD_STEP, PTD_PLAIN>(c);
+        default: return perf ? outer_step_t<PTD_NONLIN, PTD_PERF>(c) : outer_step_t<PTD_NONLIN, PTD_PLAIN>(c);
+    }
 }
 
 template <int DC, int FORM>
@@ -1298,34 +1358,6 @@ hipError_t launch_outer_residual(ptd_ctx* c) {
         case PTD_STEP: return perf ? outer_residual_t<PTD_STEP, PTD_PERF>(c) : outer_residual_t<PTD_STEP, PTD_PLAIN>(c);
         default: return perf ? outer_residual_t<PTD_NONLIN, PTD_PERF>(c) : outer_residual_t<PTD_NONLIN, PTD_PLAIN>(c);
     }
-}
-
-hipError_t launch_outer_update(ptd_ctx* c) {
-    const KP<double> kp = make_kp<double>(c->p);
-    const dim3 g(grid_for(c->nH)), bl(kBlock);
-    const double* Hin = (const double*)c->H[c->hcur];
-    double* Hout = (double*)c->H[1 - c->hcur];
-    const float* dl = (const float*)c->dl[c->dcur];
-    const double* D = (const double*)c->D;
-    float* px = (float*)c->pq[0][c->pcur];
-    float* py = (float*)c->pq[1][c->pcur];
-    float* pz = (float*)c->pq[2][c->pcur];
-    const int dc = c->p.dcase;
-    const double3 sc = make_double3(corr_flux_scale(c->p, dc, 0), corr_flux_scale(c->p, dc, 1),
-                                    corr_flux_scale(c->p, dc, 2));
-    const double3 isc = make_double3(1.0 / sc.x, 1.0 / sc.y, 1.0 / sc.z);
-    if (dc == PTD_NONLIN)
-        hipLaunchKernelGGL((k_outer_update<PTD_NONLIN>), g, bl, 0, c->stream, kp, Hin, Hout, dl, D, px, py, pz, c->nH,
-                           sc, isc);
-    else if (dc == PTD_STEP)
-        hipLaunchKernelGGL((k_outer_update<PTD_STEP>), g, bl, 0, c->stream, kp, Hin, Hout, dl, D, px, py, pz, c->nH,
-                           sc, isc);
-    else
-        hipLaunchKernelGGL((k_outer_update<PTD_LIN>), g, bl, 0, c->stream, kp, Hin, Hout, dl, D, px, py, pz, c->nH,
-                           sc, isc);
-    hipError_t e = hipGetLastError();
-    if (e == hipSuccess) c->hcur = 1 - c->hcur;
-    return e;
 }
 
 // One or two float32 correction iterations on (delta, correction fluxes) with source src = R*dt in
