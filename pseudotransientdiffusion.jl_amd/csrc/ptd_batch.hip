@@ -4,8 +4,9 @@
 // solved over nt physical steps exactly as ptd_solve would solve it alone: the same tile functions,
 // the same checking tiling (so the same residual partials and stopping decisions), the same
 // iteration counts and the same bits. What changes is the schedule:
-//  - members are grouped by (ndim, dcase, arith); one launch covers the tiles of every member of a
-//    group (a table maps each block to its member, tile and z chunk depth);
+//  - members are grouped by (sweep kind, dcase, arith), the kind being the sweep ptd_solve runs for
+//    them (two-level 3D, two-level 2D on large grids, single-level 2D); one launch covers the tiles of
+//    every member of a group (a table maps each block to its member, tile and z chunk depth);
 //  - the PT loop of every member runs on the device: after each check block of nout iterations a
 //    controller kernel applies the reference loop's rules per member (scripts/diff_3D/
 //    diff_3D_nonlin_multixpu.jl:104-118) and members that are done drop out of later launches;
@@ -38,7 +39,7 @@ const Candidate kCandidates[] = {{16, 0}, {8, 0}, {32, 0}, {16, 1}};
 constexpr int kNumCandidates = sizeof(kCandidates) / sizeof(kCandidates[0]);
 
 struct Group {
-    int ndim = 0, dcase = 0, arith = 0;
+    int kind = 0, dcase = 0, arith = 0;  // kind: the sweep of its members (ptd::batch_kind)
     std::vector<int32_t> members;
     int32_t* d_members = nullptr;
     hipStream_t stream = nullptr;
@@ -126,7 +127,9 @@ void free_batch(ptd_batch* b) {
 }
 
 // tiles of one member for a launch with `rows` rows per block and z chunk depth tz (3D)
-int member_tiles(const ptd_params& p, int rows, int tz) {
+// tiles of one member's launch: rows per block (two-level 2D: rows per tile chunk), z chunk depth tz
+int member_tiles(const ptd_params& p, int kind, int rows, int tz) {
+    if (kind == ptd::kSweep2D2) return (int)(((p.nx - 2 + 59) / 60) * ((p.ny - 2 + rows - 1) / rows));
     if (p.ndim == 3) {
         const int64_t ntx = (p.nx - 2 + 59) / 60, nty = (p.ny - 2 + rows - 5) / (rows - 4);
         const int64_t nbz = (p.nz - 2 + tz - 1) / tz;
@@ -153,12 +156,12 @@ int enqueue_block(ptd_batch* b, int cand) {
     for (size_t gi = 0; gi < b->groups.size(); ++gi) {
         Group& g = b->groups[gi];
         PTD_HIPB(hipStreamWaitEvent(g.stream, b->ev_fork, 0));
-        const int lpb = g.ndim == 3 ? b->nout / 2 : b->nout;
+        const int lpb = g.kind == ptd::kSweep2D ? b->nout : b->nout / 2;
         for (int li = 0; li < lpb; ++li) {
             const bool check = li == lpb - 1;
-            const int rows = check ? ptd::batch_rows_check(g.ndim, g.dcase, g.arith)
-                                   : ptd::batch_rows_nocheck(g.ndim, g.dcase, g.arith, kCandidates[cand].rows2d);
-            PTD_HIPB(ptd::launch_batch_sweep(g.stream, g.ndim, g.dcase, g.arith, rows, check, b->dm,
+            const int rows = check ? ptd::batch_rows_check(g.kind, g.dcase, g.arith)
+                                   : ptd::batch_rows_nocheck(g.kind, g.dcase, g.arith, kCandidates[cand].rows2d);
+            PTD_HIPB(ptd::launch_batch_sweep(g.stream, g.kind, g.dcase, g.arith, rows, check, b->dm,
                                              check ? g.d_check : g.d_nc[cand], check ? g.n_check : g.n_nc[cand], li));
         }
         PTD_HIPB(ptd::launch_batch_reduce(g.stream, b->dm, g.d_members, (int)g.members.size()));
@@ -273,7 +276,8 @@ int32_t ptd_batch_create(const ptd_params* params, int32_t n, int32_t device, pt
         if (st == PTD_OK && p.dcase == PTD_STEP) st = balloc(b, (void**)&M.D, (size_t)M.nH * 8);
         for (int a = 0; a < 3 && st == PTD_OK; ++a)
             for (int k = 0; k < 2 && st == PTD_OK; ++k) st = balloc(b, (void**)&M.q[a][k], (size_t)nq[a] * 8);
-        M.ntiles_check = member_tiles(p, ptd::batch_rows_check((int)p.ndim, p.dcase, p.arith), check_tz(p));
+        M.ntiles_check = member_tiles(p, ptd::batch_kind(p), ptd::batch_rows_check(ptd::batch_kind(p), p.dcase, p.arith),
+                                      check_tz(p));
         if (st == PTD_OK) st = balloc(b, (void**)&M.partial, (size_t)M.ntiles_check * sizeof(double2));
         if (st != PTD_OK) return fail_free(st);
         M.itmax = p.itmax;
@@ -281,7 +285,7 @@ int32_t ptd_batch_create(const ptd_params* params, int32_t n, int32_t device, pt
         M.dcase = p.dcase;
         M.arith = p.arith;
         M.nout = p.nout;
-        M.lpb = p.ndim == 3 ? p.nout / 2 : p.nout;
+        M.lpb = ptd::batch_kind(p) == ptd::kSweep2D ? p.nout : p.nout / 2;
         M.stop_norm = p.stop_norm;
         M.tol = p.tol;
         M.norm_len = p.norm_len > 0 ? p.norm_len : (double)(mx * my * mz);
@@ -291,11 +295,11 @@ int32_t ptd_batch_create(const ptd_params* params, int32_t n, int32_t device, pt
         const ptd_params& p = params[m];
         Group* g = nullptr;
         for (auto& x : b->groups)
-            if (x.ndim == p.ndim && x.dcase == p.dcase && x.arith == p.arith) g = &x;
+            if (x.kind == ptd::batch_kind(p) && x.dcase == p.dcase && x.arith == p.arith) g = &x;
         if (!g) {
             b->groups.emplace_back();
             g = &b->groups.back();
-            g->ndim = p.ndim;
+            g->kind = ptd::batch_kind(p);
             g->dcase = p.dcase;
             g->arith = p.arith;
         }
@@ -315,8 +319,9 @@ int32_t ptd_batch_create(const ptd_params* params, int32_t n, int32_t device, pt
         std::vector<BatchTile> t;
         for (int32_t m : g.members) {
             const ptd_params& p = params[m];
-            const int tz = p.ndim == 3 ? check_tz(p) : 0;
-            const int nt = member_tiles(p, ptd::batch_rows_check(g.ndim, g.dcase, g.arith), tz);
+            const int rows = ptd::batch_rows_check(g.kind, g.dcase, g.arith);
+            const int tz = p.ndim == 3 ? check_tz(p) : (g.kind == ptd::kSweep2D2 ? rows : 0);
+            const int nt = member_tiles(p, g.kind, rows, tz);
             for (int k = 0; k < nt; ++k) t.push_back({m, k, tz, 0});
         }
         g.n_check = (int)t.size();
@@ -324,11 +329,12 @@ int32_t ptd_batch_create(const ptd_params* params, int32_t n, int32_t device, pt
         if (st != PTD_OK) return fail_free(st);
         for (int c = 0; c < kNumCandidates; ++c) {
             t.clear();
-            const int rows = ptd::batch_rows_nocheck(g.ndim, g.dcase, g.arith, kCandidates[c].rows2d);
+            const int rows = ptd::batch_rows_nocheck(g.kind, g.dcase, g.arith, kCandidates[c].rows2d);
             for (int32_t m : g.members) {
                 const ptd_params& p = params[m];
-                const int tz = p.ndim == 3 ? (int)std::min<int64_t>(kCandidates[c].tz3d, p.nz - 2) : 0;
-                const int nt = member_tiles(p, rows, tz);
+                const int tz = p.ndim == 3 ? (int)std::min<int64_t>(kCandidates[c].tz3d, p.nz - 2)
+                                           : (g.kind == ptd::kSweep2D2 ? rows : 0);
+                const int nt = member_tiles(p, g.kind, rows, tz);
                 for (int k = 0; k < nt; ++k) t.push_back({m, k, tz, 0});
             }
             g.n_nc[c] = (int)t.size();

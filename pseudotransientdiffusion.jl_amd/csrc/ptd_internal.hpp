@@ -541,12 +541,18 @@ enum { kBatchMaxMembers = 1024 };
 int validate_params(const ptd_params* p);          // the single-problem checks (ptd_api.hip)
 int set_error(int code, const std::string& msg);   // sets ptd_last_error(), returns code
 
-// A batched sweep launch over the tiles of every member of one (ndim, dcase, arith) group: launch
-// `li` of the check block (buffers selected by the members' parities + li), with rows R per block.
-hipError_t launch_batch_sweep(hipStream_t s, int ndim, int dcase, int arith, int rows, bool check,
+// The sweep a problem runs, as ptd_solve picks it: kSweep3D (two-level 3D), kSweep2D2 (two-level 2D,
+// grids with enough tiles to fill the GPU) or kSweep2D (single-level 2D).
+enum { kSweep2D = 2, kSweep3D = 3, kSweep2D2 = 4 };
+bool use_2d_t2(const ptd_params& p);
+int batch_kind(const ptd_params& p);
+// A batched sweep launch over the tiles of every member of one (kind, dcase, arith) group: launch
+// `li` of the check block (buffers selected by the members' parities + li), with rows R per block
+// (two-level 2D: R = rows per tile chunk).
+hipError_t launch_batch_sweep(hipStream_t s, int kind, int dcase, int arith, int rows, bool check,
                               BatchMember* mem, const BatchTile* tiles, int ntiles, int li);
-int batch_rows_nocheck(int ndim, int dcase, int arith, int variant);  // rows of the non-checking launch
-int batch_rows_check(int ndim, int dcase, int arith);                                       // rows of the checking launch
+int batch_rows_nocheck(int kind, int dcase, int arith, int variant);  // rows of the non-checking launch
+int batch_rows_check(int kind, int dcase, int arith);                 // rows of the checking launch
 hipError_t launch_batch_reduce(hipStream_t s, BatchMember* mem, const int32_t* members, int n);
 hipError_t launch_batch_control(hipStream_t s, BatchMember* mem, int n, int64_t* iters, double* errs,
                                 int32_t* active_count, int nt_cap);

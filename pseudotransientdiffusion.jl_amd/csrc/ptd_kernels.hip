@@ -516,6 +516,161 @@ k_fused2d(const KP<T> p, const T* __restrict__ Hin, T* __restrict__ Hout, const 
 }
 
 // ---------------------------------------------------------------------------------------------
+// fused 2D sweep over two PT iterations (temporal blocking, one wavefront per tile)
+// ---------------------------------------------------------------------------------------------
+//
+// A tile is a strip of 64 columns (lane L holds i = 60*bx - 1 + L) marching along y over the rows of
+// its chunk; the block is that one wavefront, so the march needs no LDS and no barrier: x neighbours
+// are lane shuffles, y neighbours are the rows carried in registers. Level 1 (iteration n+1) runs one
+// row ahead of level 2 (iteration n+2): at step jj level 1 updates row jj and level 2 row jj-1, each
+// face and cell with the same expressions as the single-level 2D sweep, so two iterations give the
+// same bits as two k_fused2d launches. Level 1 is valid on lanes 1..62 and rows j0-1..j1, level 2 on
+// lanes 2..61 and rows j0..j1-1 (j0 = 1 + ty*by, j1 = min(j0 + ty, ny - 1)); each face and cell is
+// written by exactly one tile. CHECK also reduces the plain form's residual of iteration n+2.
+template <typename T, int DC, int FORM, int ARITH, bool CHECK>
+__device__ __forceinline__ void fused2d_t2_tile(const KP<T>& p, const T* __restrict__ Hin, T* __restrict__ Hout,
+                                                const T* __restrict__ Hold, const T* __restrict__ D,
+                                                const T* __restrict__ qxi, const T* __restrict__ qyi,
+                                                T* __restrict__ qxo, T* __restrict__ qyo,
+                                                double2* __restrict__ part, int bx, int by, int ty) {
+    using Ph = Phys<T, DC, FORM, ARITH>;
+    constexpr bool STEP = DC == PTD_STEP;
+    const int lane = threadIdx.x;
+    const int nx = (int)p.nx, ny = (int)p.ny;
+    const int i = 60 * bx - 1 + lane;
+    const int j0 = 1 + ty * by;
+    const int j1 = j0 + ty < ny - 1 ? j0 + ty : ny - 1;  // level 2 updates rows j0..j1-1
+    const int ja = j0 - 1;                               // level 1 runs rows ja..j1
+    const bool xin = i >= 1 && i <= nx - 2;
+    const bool l1 = lane >= 1 && lane <= 62 && xin;      // level-1 update lanes
+    const bool l2 = lane >= 2 && lane <= 61 && xin;      // level-2 update lanes (owned cells)
+    // owned x faces: lanes 2..61 inside the domain, and the face (nx-2 | nx-1) at the domain end
+    const bool wx = lane >= 2 && lane <= 62 && ((lane <= 61 && i <= nx - 2) || i == nx - 1);
+    const bool wy = l2;                                  // owned y faces: the lower face of each owned cell
+
+    const int ic = max(0, min(i, nx - 1));
+    const int ixq = max(1, min(i, nx - 1)), iyq = max(1, min(i, nx - 2));
+    auto row_h = [&](int r) { return (int64_t)ic + (int64_t)nx * r; };            // H, Hold, D of row r
+    auto row_qx = [&](int r) {                                                   // x face (i-1 | i) of row r
+        const int rc = max(1, min(r, ny - 2));
+        return (int64_t)(ixq - 1) + (int64_t)(nx - 1) * (rc - 1);
+    };
+    auto row_qy = [&](int f) {                                                   // y face (f | f+1)
+        const int fc = max(0, min(f, ny - 2));
+        return (int64_t)(iyq - 1) + (int64_t)(nx - 2) * fc;
+    };
+
+    // rows ja-1 .. : H(jj+1) is loaded one row ahead
+    const int rb = ja > 0 ? ja - 1 : 0;
+    T Hc = Hin[row_h(ja)];
+    T Hn = Hin[row_h(ja + 1 < ny ? ja + 1 : ny - 1)];
+    T Dp = 0, Dc = 0, Dn = 0, Dpp = 0;  // D at rows jj-1, jj, jj+1, jj-2
+    if constexpr (STEP) {
+        Dp = D[row_h(rb)];
+        Dc = D[row_h(ja)];
+        Dn = D[row_h(ja + 1 < ny ? ja + 1 : ny - 1)];
+    }
+    T unused = 0;
+    // level-1 lower y face of row ja: face (ja-1 | ja), from H(ja-1), H(ja) (needed when ja is inner)
+    T q1yL = Ph::template face<false>(p, qyi[row_qy(ja - 1)], Hin[row_h(rb)], Hc, Dp, Dc, p.dy, p._dy, p._dy8,
+                                      unused);
+    T H1p = 0, Hop = 0, q1xp = 0, q2yL = 0, r2yL = 0;
+    T acc = 0, amax = 0;
+
+    for (int jj = ja; jj <= j1; ++jj) {
+        const bool rin = jj >= 1 && jj <= ny - 2;
+        // operands of row jj (and of the next row for the upper y face)
+        const T Ho = Hold[row_h(jj)];
+        const T qxo_ = qxi[row_qx(jj)];
+        const T qyo_ = qyi[row_qy(jj)];
+        const T Hnn = Hin[row_h(jj + 2 < ny ? jj + 2 : ny - 1)];  // prefetch for the next step
+        T Dnn = 0;
+        if constexpr (STEP) Dnn = D[row_h(jj + 2 < ny ? jj + 2 : ny - 1)];
+
+        // level 1, row jj
+        const T Hw = __shfl_up(Hc, 1);
+        T Dw = 0, De = 0;
+        if constexpr (STEP) {
+            Dw = __shfl_up(Dc, 1);
+            De = __shfl_down(Dc, 1);
+        }
+        const T q1x = Ph::template face<false>(p, qxo_, Hw, Hc, Dw, Dc, p.dx, p._dx, p._dx8, unused);
+        const T q1yU = Ph::template face<false>(p, qyo_, Hc, Hn, Dc, Dn, p.dy, p._dy, p._dy8, unused);
+        const T q1xE = __shfl_down(q1x, 1);
+        T dmax1 = 0;
+        if constexpr (STEP) {
+            dmax1 = fmax(fmax(Dw, De), Dc);
+            dmax1 = fmax(dmax1, fmax(Dp, Dn));
+        }
+        const T dv1 = Ph::template div<2>(p, q1xE - q1x, q1yU - q1yL, (T)0);
+        const T H1c = Ph::cell(p, Hc, Ho, dv1, dmax1, Dc);
+        const T H1 = (l1 && rin) ? H1c : Hc;
+
+        // level 2: upper y face of row jj-1 (face (jj-1 | jj)) and row jj-1 itself
+        T r2yU = 0;
+        const T q2yU = Ph::template face<CHECK>(p, q1yL, H1p, H1, Dp, Dc, p.dy, p._dy, p._dy8, r2yU);
+        const int m = jj - 1;
+        if (m >= j0) {
+            const T H1w = __shfl_up(H1p, 1);
+            T Dpw = 0, Dpe = 0;
+            if constexpr (STEP) {
+                Dpw = __shfl_up(Dp, 1);
+                Dpe = __shfl_down(Dp, 1);
+            }
+            T r2x = 0;
+            const T q2x = Ph::template face<CHECK>(p, q1xp, H1w, H1p, Dpw, Dp, p.dx, p._dx, p._dx8, r2x);
+            const T q2xE = __shfl_down(q2x, 1);
+            T dmax2 = 0;
+            if constexpr (STEP) {
+                dmax2 = fmax(fmax(Dpw, Dpe), Dp);
+                dmax2 = fmax(dmax2, fmax(Dpp, Dc));
+            }
+            const T dv2 = Ph::template div<2>(p, q2xE - q2x, q2yU - q2yL, (T)0);
+            const T H2 = Ph::cell(p, H1p, Hop, dv2, dmax2, Dp);
+            if constexpr (CHECK) {
+                const T r2xE = __shfl_down(r2x, 1);
+                const T dvr = Ph::template div<2>(p, r2xE - r2x, r2yU - r2yL, (T)0);
+                const T rr = l2 ? Ph::resid(p, H2, Hop, dvr) : (T)0;
+                acc += rr * rr;
+                amax = nanmax(fabs(rr), amax);
+            }
+            if (l2) Hout[row_h(m)] = H2;
+            if (wx) qxo[row_qx(m)] = q2x;
+            if (wy) qyo[row_qy(m - 1)] = q2yL;
+            if (wy && m == ny - 2) qyo[row_qy(m)] = q2yU;  // face (ny-2 | ny-1): the domain end
+        }
+        // carry to row jj+1
+        q1yL = q1yU;
+        q2yL = q2yU;
+        r2yL = r2yU;
+        H1p = H1;
+        Hop = Ho;
+        q1xp = q1x;
+        Hc = Hn;
+        Hn = Hnn;
+        if constexpr (STEP) {
+            Dpp = Dp;
+            Dp = Dc;
+            Dc = Dn;
+            Dn = Dnn;
+        }
+    }
+    if constexpr (CHECK) block_sum_max((double)acc, (double)amax, lane, 64, part);
+}
+
+template <typename T, int DC, int FORM, int ARITH, bool CHECK>
+__global__ void __launch_bounds__(64)
+k_fused2d_t2(const KP<T> p, const T* __restrict__ Hin, T* __restrict__ Hout, const T* __restrict__ Hold,
+             const T* __restrict__ D, const T* __restrict__ qxi, const T* __restrict__ qyi, T* __restrict__ qxo,
+             T* __restrict__ qyo, double2* __restrict__ partial, int ty, int ntx, int ntiles, int per_xcd) {
+    const int b = blockIdx.x;
+    const int tile = (b & 7) * per_xcd + (b >> 3);  // XCD-aware: each XCD walks a contiguous tile range
+    if (tile >= ntiles) return;
+    fused2d_t2_tile<T, DC, FORM, ARITH, CHECK>(p, Hin, Hout, Hold, D, qxi, qyi, qxo, qyo, &partial[tile],
+                                               tile % ntx, tile / ntx, ty);
+}
+
+// ---------------------------------------------------------------------------------------------
 // fused 3D sweep over two PT iterations (temporal blocking)
 // ---------------------------------------------------------------------------------------------
 //
@@ -1023,6 +1178,41 @@ static hipError_t sweep2d_run(ptd_ctx* c, const SweepBufs& b, bool check) {
     return hipGetLastError();
 }
 
+static constexpr int kTy2D = 16;  // rows per tile (y chunk) of the two-level 2D sweep
+
+// One two-level 2D pass (two iterations); with `check` the plain-form residual of the second is
+// reduced into c->dred.
+template <typename T, int DC, int FORM, int ARITH>
+static hipError_t sweep2d_t2_run(ptd_ctx* c, const SweepBufs& b, bool check) {
+    const KP<T> kp = make_kp<T>(c->p);
+    const int ntx = (int)((c->p.nx - 2 + 59) / 60);
+    const int nty = (int)((c->p.ny - 2 + kTy2D - 1) / kTy2D);
+    const int ntiles = ntx * nty;
+    const int per = (ntiles + 7) / 8;
+    const dim3 g(8 * per), bl(64);
+    const T* Hin = (const T*)b.Hin;
+    T* Hout = (T*)b.Hout;
+    const T* Ho = (const T*)b.Hold;
+    const T* A = (const T*)b.aux;
+    const T* q0 = (const T*)b.qin[0];
+    const T* q1 = (const T*)b.qin[1];
+    T* o0 = (T*)b.qout[0];
+    T* o1 = (T*)b.qout[1];
+    double2* part = (double2*)c->partial;
+    if (check)
+        hipLaunchKernelGGL((k_fused2d_t2<T, DC, FORM, ARITH, true>), g, bl, 0, c->stream, kp, Hin, Hout, Ho, A, q0,
+                           q1, o0, o1, part, kTy2D, ntx, ntiles, per);
+    else
+        hipLaunchKernelGGL((k_fused2d_t2<T, DC, FORM, ARITH, false>), g, bl, 0, c->stream, kp, Hin, Hout, Ho, A,
+                           q0, q1, o0, o1, part, kTy2D, ntx, ntiles, per);
+    hipError_t e = hipGetLastError();
+    if (e != hipSuccess) return e;
+    if (check)
+        hipLaunchKernelGGL(k_reduce_partials, dim3(1), dim3(kBlock), 0, c->stream, (const double2*)c->partial,
+                           (int64_t)ntiles, c->dred);
+    return hipGetLastError();
+}
+
 // The context's own fields (H, q double-buffered; the sweep flips them).
 static SweepBufs ctx_bufs(ptd_ctx* c) {
     const int h = c->hcur, q = c->qcur;
@@ -1042,7 +1232,10 @@ template <typename T, int DC, int FORM, bool TWO>
 static hipError_t ctx_sweep_t(ptd_ctx* c, bool check) {
     const SweepBufs b = ctx_bufs(c);
     hipError_t e;
-    if (c->p.ndim == 2)
+    if (c->p.ndim == 2 && TWO)
+        e = c->p.arith == PTD_ARITH_FAST ? sweep2d_t2_run<T, DC, FORM, kFast>(c, b, check)
+                                         : sweep2d_t2_run<T, DC, FORM, kExact>(c, b, check);
+    else if (c->p.ndim == 2)
         e = c->p.arith == PTD_ARITH_FAST ? sweep2d_run<T, DC, FORM, kFast>(c, b, check)
                                          : sweep2d_run<T, DC, FORM, kExact>(c, b, check);
     else
@@ -1107,7 +1300,18 @@ bool fused_available(const ptd_ctx* c) {
     return (c->p.ndim == 3 && c->p.nx >= 3 && c->p.ny >= 3 && c->p.nz >= 3 && max_chunk_tz(c->p, (int64_t)c->esz) >= 1) ||
            (c->p.ndim == 2 && c->p.nx >= 3 && c->p.ny >= 3);
 }
-bool fused2_available(const ptd_ctx* c) { return c->p.ndim == 3 && fused_available(c); }
+// Two-level 2D sweeps march one wavefront per 60-column x 16-row tile: below ~1024 tiles they cannot
+// fill the GPU and the single-level sweep (16-wave blocks) is faster (measured: 512^2 7.4 vs 9.6 us per
+// iteration single / two-level, 1024^2 14.4 vs 11.2, 2048^2 40.6 vs 26.0).
+bool use_2d_t2(const ptd_params& p) {
+    if (p.ndim != 2) return false;
+    const int64_t ntx = (p.nx - 2 + 59) / 60, nty = (p.ny - 2 + kTy2D - 1) / kTy2D;
+    return ntx * nty >= 1024;
+}
+int batch_kind(const ptd_params& p) { return p.ndim == 3 ? kSweep3D : (use_2d_t2(p) ? kSweep2D2 : kSweep2D); }
+bool fused2_available(const ptd_ctx* c) {
+    return fused_available(c) && (c->p.ndim == 3 || use_2d_t2(c->p));
+}
 
 // ---------------------------------------------------------------------------------------------
 // mixed precision: float32 correction sweeps around a float64 iterate
@@ -1395,6 +1599,8 @@ int64_t partial_capacity(const ptd_params& p, int64_t nR) {
     if (p.ndim == 2) {
         const int64_t ntx = (p.nx - 2 + 61) / 62, nty = (p.ny - 2 + kRows2D - 3) / (kRows2D - 2);
         nb = std::max<int64_t>(nb, ntx * nty);
+        const int64_t ntx2 = (p.nx - 2 + 59) / 60, nty2 = (p.ny - 2 + kTy2D - 1) / kTy2D;
+        nb = std::max<int64_t>(nb, ntx2 * nty2);
     }
     if (p.ndim == 3) {
         const int64_t ntx = (p.nx - 2 + 61) / 62, nty = (p.ny - 2 + kRows - 3) / (kRows - 2);
@@ -1557,25 +1763,53 @@ k_batch2d(BatchMember* __restrict__ mem, const BatchTile* __restrict__ tiles, in
         tile % ntx, tile / ntx);
 }
 
+template <int DC, int ARITH, bool CHECK>
+__global__ void __launch_bounds__(64)
+k_batch2d_t2(BatchMember* __restrict__ mem, const BatchTile* __restrict__ tiles, int ntot, int per_xcd, int li) {
+    const int b = blockIdx.x;
+    const int t = (b & 7) * per_xcd + (b >> 3);
+    if (t >= ntot) return;
+    const BatchTile e = tiles[t];
+    const int mi = __builtin_amdgcn_readfirstlane(e.member);
+    const int tile = __builtin_amdgcn_readfirstlane(e.tile);
+    const int ty = __builtin_amdgcn_readfirstlane(e.tz);
+    const BatchMember& M = mem[mi];
+    if (!M.active) return;
+    const int ntx = ((int)M.p.nx - 2 + 59) / 60;
+    const int hp = (M.hpar + li) & 1, qp = (M.qpar + li) & 1;
+    fused2d_t2_tile<double, DC, PTD_PLAIN, ARITH, CHECK>(
+        M.p, as_global(M.H[hp]), as_global(M.H[hp ^ 1]), as_global(M.Hold), as_global(M.D), as_global(M.q[0][qp]),
+        as_global(M.q[1][qp]), as_global(M.q[0][qp ^ 1]), as_global(M.q[1][qp ^ 1]), as_global(M.partial + tile),
+        tile % ntx, tile / ntx, ty);
+}
+
 // rows of the batched launches: the checking launch uses the single-problem checking tiling (so the
 // residual partials, and hence every stopping decision, are those of a single-problem solve); the
 // other launches may use any tiling (every cell is computed by its owner with the same expressions).
-int batch_rows_check(int ndim, int dcase, int arith) {
-    if (ndim != 3) return kRows2D;
+int batch_rows_check(int kind, int dcase, int arith) {
+    if (kind == kSweep2D2) return kTy2D;
+    if (kind != kSweep3D) return kRows2D;
     if (arith != PTD_ARITH_FAST) return t2_rows<double, PTD_NONLIN, kExact, true>();
     return dcase == PTD_STEP ? t2_rows<double, PTD_STEP, kFast, true>() : t2_rows<double, PTD_NONLIN, kFast, true>();
 }
-int batch_rows_nocheck(int ndim, int dcase, int arith, int variant) {
-    if (ndim == 2) return variant == 1 ? 8 : 16;
+int batch_rows_nocheck(int kind, int dcase, int arith, int variant) {
+    if (kind == kSweep2D2) return kTy2D;
+    if (kind == kSweep2D) return variant == 1 ? 8 : 16;
     return (arith == PTD_ARITH_FAST && dcase != PTD_STEP) ? 12 : 16;
 }
 
 template <int DC, int ARITH>
-static hipError_t batch_sweep_t(hipStream_t s, int ndim, int rows, bool check, BatchMember* mem,
+static hipError_t batch_sweep_t(hipStream_t s, int kind, int rows, bool check, BatchMember* mem,
                                 const BatchTile* tiles, int ntiles, int li) {
     const int per = (ntiles + 7) / 8;
     const dim3 g(8 * per);
-    if (ndim == 3) {
+    if (kind == kSweep2D2) {
+        if (rows != kTy2D) return hipErrorInvalidValue;
+        if (check)
+            hipLaunchKernelGGL((k_batch2d_t2<DC, ARITH, true>), g, dim3(64), 0, s, mem, tiles, ntiles, per, li);
+        else
+            hipLaunchKernelGGL((k_batch2d_t2<DC, ARITH, false>), g, dim3(64), 0, s, mem, tiles, ntiles, per, li);
+    } else if (kind == kSweep3D) {
         constexpr int RN = t2_rows<double, DC, ARITH, false>(), RC = t2_rows<double, DC, ARITH, true>();
         if (check) {
             if (rows != RC) return hipErrorInvalidValue;
@@ -1599,20 +1833,20 @@ static hipError_t batch_sweep_t(hipStream_t s, int ndim, int rows, bool check, B
     return hipGetLastError();
 }
 
-hipError_t launch_batch_sweep(hipStream_t s, int ndim, int dcase, int arith, int rows, bool check,
+hipError_t launch_batch_sweep(hipStream_t s, int kind, int dcase, int arith, int rows, bool check,
                               BatchMember* mem, const BatchTile* tiles, int ntiles, int li) {
     if (ntiles <= 0) return hipSuccess;
     const bool fast = arith == PTD_ARITH_FAST;
     switch (dcase) {
         case PTD_LIN:
-            return fast ? batch_sweep_t<PTD_LIN, kFast>(s, ndim, rows, check, mem, tiles, ntiles, li)
-                        : batch_sweep_t<PTD_LIN, kExact>(s, ndim, rows, check, mem, tiles, ntiles, li);
+            return fast ? batch_sweep_t<PTD_LIN, kFast>(s, kind, rows, check, mem, tiles, ntiles, li)
+                        : batch_sweep_t<PTD_LIN, kExact>(s, kind, rows, check, mem, tiles, ntiles, li);
         case PTD_STEP:
-            return fast ? batch_sweep_t<PTD_STEP, kFast>(s, ndim, rows, check, mem, tiles, ntiles, li)
-                        : batch_sweep_t<PTD_STEP, kExact>(s, ndim, rows, check, mem, tiles, ntiles, li);
+            return fast ? batch_sweep_t<PTD_STEP, kFast>(s, kind, rows, check, mem, tiles, ntiles, li)
+                        : batch_sweep_t<PTD_STEP, kExact>(s, kind, rows, check, mem, tiles, ntiles, li);
         default:
-            return fast ? batch_sweep_t<PTD_NONLIN, kFast>(s, ndim, rows, check, mem, tiles, ntiles, li)
-                        : batch_sweep_t<PTD_NONLIN, kExact>(s, ndim, rows, check, mem, tiles, ntiles, li);
+            return fast ? batch_sweep_t<PTD_NONLIN, kFast>(s, kind, rows, check, mem, tiles, ntiles, li)
+                        : batch_sweep_t<PTD_NONLIN, kExact>(s, kind, rows, check, mem, tiles, ntiles, li);
     }
 }
 

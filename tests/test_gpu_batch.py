@@ -17,6 +17,8 @@ MEMBERS = [
     ("diff_3D_linstep_multixpu", (30, 24, 20), 2.0, (0.5, 0.5, 0.5)),
     ("diff_3D_nonlin_multixpu", (28, 28, 28), 1.0, (0.45, 0.55, 0.5)),
     ("diff_3D_nonlin_multixpu", (67, 35, 21), 0.7, (0.3, 0.7, 0.5)),
+    # large enough (17 x 62 tiles) for the two-level 2D sweep
+    ("diff_2D_nonlin", (1000, 990), 1.0, (0.5, 0.45)),
 ]
 
 

@@ -370,6 +370,72 @@ def test_two_level_equals_single_sweeps_fast(script, pkg, orc, monkeypatch):
         assert np.array_equal(a, b)
 
 
+# Two-level 2D sweeps run on grids with at least 1024 tiles of 60 x 16 cells; (1000, 990) has 17 x 62.
+T2D_SCRIPTS = ["diff_2D_lin", "diff_2D_linstep", "diff_2D_nonlin", "diff_2D_nonlin_perf"]
+T2D_SHAPE = (1000, 990)
+
+
+@pytest.mark.parametrize("script", T2D_SCRIPTS)
+def test_two_level_2d_sweeps_match_oracle(script, pkg, orc, monkeypatch):
+    """EXACT arithmetic, 2D: three two-level passes plus one single sweep give the oracle's values bit for
+    bit (H and both fluxes)."""
+    niter = 7
+    print(f"[gpu] t2 2D {script} {T2D_SHAPE} niter={niter}", flush=True)
+    monkeypatch.setenv("PTD_T2", "1")
+    params, case, H, Hold, D, q = _random_state(pkg, orc, script, T2D_SHAPE, seed=11)
+    with pkg.Context(params) as ctx:
+        ctx.set("H", H)
+        ctx.set("Hold", Hold)
+        for k, v in q.items():
+            ctx.set(k, v)
+        dtau = None
+        if D is not None:
+            ctx.set("D", D)
+            dtau = np.zeros(pkg.field_shape(params, 10), order="F")
+            orc.lib().ptdo_iter_params(orc.ctypes.byref(case), orc._p(D), orc._p(dtau))
+        ctx.iterate(niter)
+        Hg = ctx.get("H")
+        qg = {k: ctx.get(k) for k in q}
+    Hr = H.copy(order="F")
+    qr = {k: v.copy(order="F") for k, v in q.items()}
+    orc.lib().ptdo_iterate(orc.ctypes.byref(case), orc._p(Hr), orc._p(Hold), orc._p(D), orc._p(dtau),
+                           orc._p(qr["qHx"]), orc._p(qr["qHy"]), None, niter)
+    assert np.all(np.isfinite(Hr))
+    assert np.array_equal(Hg, Hr), f"H: rel {_relmax(Hg, Hr):.2e}"
+    for k in q:
+        assert np.array_equal(qg[k], qr[k]), f"{k}: rel {_relmax(qg[k], qr[k]):.2e}"
+
+
+@pytest.mark.parametrize("script", ["diff_2D_lin", "diff_2D_linstep", "diff_2D_nonlin"])
+@pytest.mark.parametrize("precision", [0, 1], ids=["f64", "f32"])
+def test_two_level_2d_converged_step_equals_single_level(script, precision, pkg, orc, monkeypatch):
+    """FAST arithmetic, 2D: one converged physical step with two-level sweeps (and their checking variant)
+    takes the same iterations as with single sweeps and ends on the same bits; the residuals agree to
+    summation order."""
+    print(f"[gpu] t2 2D converged step {script} precision={precision}", flush=True)
+    spec = pkg.SCRIPTS[script]
+    out = []
+    for t2 in ("1", "0"):
+        monkeypatch.setenv("PTD_T2", t2)
+        params = pkg.make_params(spec, *T2D_SHAPE, precision=precision, arith=1)
+        if precision == 1:
+            params.itmax = 200  # float32 residuals stall above 1e-8: a bounded run with checks
+        H0 = pkg.initial_condition(spec, *T2D_SHAPE)
+        with pkg.Context(params) as ctx:
+            ctx.set("H", H0)
+            ctx.set("Hold", H0)
+            if spec["dcase"] == pkg.STEP:
+                ctx.set("D", pkg.step_field(spec, *T2D_SHAPE))
+            it, rms, mx = ctx.pt_solve()
+            out.append((it, rms, ctx.get("H"), [ctx.get(k) for k in ("qHx", "qHy")]))
+    assert out[0][0] == out[1][0] and out[0][0] > 0
+    rtol = 1e-12 if precision == 0 else 1e-5  # per-thread partial sums in the sweep's precision
+    assert abs(out[0][1] - out[1][1]) <= rtol * abs(out[1][1]) + 1e-300
+    assert np.array_equal(out[0][2], out[1][2])
+    for a, b in zip(out[0][3], out[1][3]):
+        assert np.array_equal(a, b)
+
+
 # --------------------------------------------------------------------------------------------------
 # mixed precision: float32 correction sweeps inside float64 defect correction
 # --------------------------------------------------------------------------------------------------
