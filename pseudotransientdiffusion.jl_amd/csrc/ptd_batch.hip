@@ -47,7 +47,16 @@ struct Group {
     int n_check = 0;
     BatchTile* d_nc[kNumCandidates] = {};
     int n_nc[kNumCandidates] = {};
+    BatchTile* d_t2 = nullptr;  // single-level 2D groups: two-level tiles of their pair launches
+    int n_t2 = 0;
 };
+
+// Launches per check block of nout iterations: two-level sweeps take two iterations per launch. The
+// single-level 2D groups (grids too small for ptd_solve to pick the two-level 2D sweep) still run
+// their first nout - 2 iterations as two-level pairs, which give the same bits; only the checking
+// launch must use the single-problem tiling (its residual partials decide when a member stops), and it
+// follows one single-level launch.
+int launches_per_block(int kind, int nout) { return kind == ptd::kSweep2D ? nout / 2 + 1 : nout / 2; }
 
 }  // namespace
 
@@ -156,9 +165,15 @@ int enqueue_block(ptd_batch* b, int cand) {
     for (size_t gi = 0; gi < b->groups.size(); ++gi) {
         Group& g = b->groups[gi];
         PTD_HIPB(hipStreamWaitEvent(g.stream, b->ev_fork, 0));
-        const int lpb = g.kind == ptd::kSweep2D ? b->nout : b->nout / 2;
+        const int lpb = launches_per_block(g.kind, b->nout);
         for (int li = 0; li < lpb; ++li) {
             const bool check = li == lpb - 1;
+            if (g.kind == ptd::kSweep2D && li < lpb - 2) {  // a two-level pair
+                PTD_HIPB(ptd::launch_batch_sweep(g.stream, ptd::kSweep2D2, g.dcase, g.arith,
+                                                 ptd::batch_rows_nocheck(ptd::kSweep2D2, g.dcase, g.arith, 0), false,
+                                                 b->dm, g.d_t2, g.n_t2, li));
+                continue;
+            }
             const int rows = check ? ptd::batch_rows_check(g.kind, g.dcase, g.arith)
                                    : ptd::batch_rows_nocheck(g.kind, g.dcase, g.arith, kCandidates[cand].rows2d);
             PTD_HIPB(ptd::launch_batch_sweep(g.stream, g.kind, g.dcase, g.arith, rows, check, b->dm,
@@ -285,7 +300,7 @@ int32_t ptd_batch_create(const ptd_params* params, int32_t n, int32_t device, pt
         M.dcase = p.dcase;
         M.arith = p.arith;
         M.nout = p.nout;
-        M.lpb = ptd::batch_kind(p) == ptd::kSweep2D ? p.nout : p.nout / 2;
+        M.lpb = launches_per_block(ptd::batch_kind(p), p.nout);
         M.stop_norm = p.stop_norm;
         M.tol = p.tol;
         M.norm_len = p.norm_len > 0 ? p.norm_len : (double)(mx * my * mz);
@@ -339,6 +354,17 @@ int32_t ptd_batch_create(const ptd_params* params, int32_t n, int32_t device, pt
             }
             g.n_nc[c] = (int)t.size();
             st = upload_table(b, t, &g.d_nc[c]);
+            if (st != PTD_OK) return fail_free(st);
+        }
+        if (g.kind == ptd::kSweep2D) {
+            t.clear();
+            const int ty = ptd::batch_rows_nocheck(ptd::kSweep2D2, g.dcase, g.arith, 0);
+            for (int32_t m : g.members) {
+                const int nt = member_tiles(params[m], ptd::kSweep2D2, ty, ty);
+                for (int k = 0; k < nt; ++k) t.push_back({m, k, ty, 0});
+            }
+            g.n_t2 = (int)t.size();
+            st = upload_table(b, t, &g.d_t2);
             if (st != PTD_OK) return fail_free(st);
         }
     }
