@@ -64,6 +64,23 @@ __device__ __forceinline__ void block_sum_max(double s, double m, int tid, int n
 // Q2MODE 0: damped flux only (perf compute_flux!); 1: damped and residual flux from the same H
 // (plain compute_flux!, scripts/diff_3D/diff_3D_nonlin_multixpu.jl:37-45); 2: residual flux only
 // (compute_flux_res!, scripts/diff_3D/diff_3D_nonlin_perf.jl:44-49).
+// t = a + n1*(b + n2*c) -> (a, b, c): 32-bit divisions while t and the sizes fit in 32 bits (every
+// grid below 2^32 elements; a 64-bit division costs several times more instructions), 64-bit otherwise.
+__device__ __forceinline__ void split3(int64_t t, int64_t n1, int64_t n2, int64_t& a, int64_t& b, int64_t& c) {
+    if (t <= 0xffffffffLL && n1 <= 0xffffffffLL && n2 <= 0xffffffffLL) {
+        const uint32_t tu = (uint32_t)t, n1u = (uint32_t)n1, n2u = (uint32_t)n2;
+        const uint32_t r = tu / n1u, cu = r / n2u;
+        a = (int64_t)(tu - r * n1u);
+        b = (int64_t)(r - cu * n2u);
+        c = (int64_t)cu;
+    } else {
+        const int64_t r = t / n1;
+        a = t - r * n1;
+        c = r / n2;
+        b = r - c * n2;
+    }
+}
+
 template <typename T, int DC, int FORM, int AXIS, int Q2MODE>
 __global__ void __launch_bounds__(kBlock)
 k_flux(const KP<T> p, const T* __restrict__ H, const T* __restrict__ D, T* __restrict__ q,
@@ -73,19 +90,22 @@ k_flux(const KP<T> p, const T* __restrict__ H, const T* __restrict__ D, T* __res
     int64_t a, step;
     T h, ih;
     if constexpr (AXIS == 0) {
-        const int64_t nf = p.nx - 1, f = t % nf, r = t / nf, jj = r % p.my, kk = r / p.my;
+        int64_t f, jj, kk;
+        split3(t, p.nx - 1, p.my, f, jj, kk);
         a = f + p.nx * ((jj + p.hy) + p.ny * (kk + p.hz));
         step = 1;
         h = p.dx;
         ih = p._dx;
     } else if constexpr (AXIS == 1) {
-        const int64_t nf = p.ny - 1, ii = t % p.mx, r = t / p.mx, f = r % nf, kk = r / nf;
+        int64_t ii, f, kk;
+        split3(t, p.mx, p.ny - 1, ii, f, kk);
         a = (ii + 1) + p.nx * (f + p.ny * (kk + p.hz));
         step = p.nx;
         h = p.dy;
         ih = p._dy;
     } else {
-        const int64_t ii = t % p.mx, r = t / p.mx, jj = r % p.my, f = r / p.my;
+        int64_t ii, jj, f;
+        split3(t, p.mx, p.my, ii, jj, f);
         a = (ii + 1) + p.nx * ((jj + 1) + p.ny * f);
         step = p.nx * p.ny;
         h = p.dz;
@@ -116,7 +136,8 @@ struct CellIdx {
 
 template <typename T>
 __device__ __forceinline__ CellIdx cell_index(const KP<T>& p, int64_t t) {
-    const int64_t ii = t % p.mx, r = t / p.mx, jj = r % p.my, kk = r / p.my;
+    int64_t ii, jj, kk;
+    split3(t, p.mx, p.my, ii, jj, kk);
     CellIdx c;
     c.h = (ii + 1) + p.nx * ((jj + p.hy) + p.ny * (kk + p.hz));
     c.x0 = ii + (p.nx - 1) * (jj + p.my * kk);
@@ -1341,17 +1362,20 @@ k_flux_state(const KP<double> p, const double* __restrict__ H, const double* __r
     int64_t a, step;
     double ih;
     if constexpr (AXIS == 0) {
-        const int64_t nf = p.nx - 1, f = t % nf, r = t / nf, jj = r % p.my, kk = r / p.my;
+        int64_t f, jj, kk;
+        split3(t, p.nx - 1, p.my, f, jj, kk);
         a = f + p.nx * ((jj + p.hy) + p.ny * (kk + p.hz));
         step = 1;
         ih = p._dx;
     } else if constexpr (AXIS == 1) {
-        const int64_t nf = p.ny - 1, ii = t % p.mx, r = t / p.mx, f = r % nf, kk = r / nf;
+        int64_t ii, f, kk;
+        split3(t, p.mx, p.ny - 1, ii, f, kk);
         a = (ii + 1) + p.nx * (f + p.ny * (kk + p.hz));
         step = p.nx;
         ih = p._dy;
     } else {
-        const int64_t ii = t % p.mx, r = t / p.mx, jj = r % p.my, f = r / p.my;
+        int64_t ii, jj, f;
+        split3(t, p.mx, p.my, ii, jj, f);
         a = (ii + 1) + p.nx * ((jj + 1) + p.ny * f);
         step = p.nx * p.ny;
         ih = p._dz;
@@ -1417,7 +1441,8 @@ k_outer_residual(const KP<double> p, const double* __restrict__ H, const double*
         if (hcf) hcf[g] = (float)Hc;
         dl[g] = 0.0f;
         const int64_t nx = p.nx, ny = p.ny, pl = nx * ny;
-        const int64_t i = g % nx, rest = g / nx, j = rest % ny, k = rest / ny;
+        int64_t i, j, k;
+        split3(g, nx, ny, i, j, k);
         if (i >= 1 && i <= nx - 2 && j >= 1 && j <= ny - 2 && k >= 1 && k <= p.nz - 2) {
             auto Dv = [&](int64_t o) { return DC == PTD_STEP ? D[o] : 0.0; };
             const double Dc = Dv(g);
@@ -1457,7 +1482,8 @@ k_outer_step(const KP<double> p, const double* __restrict__ Hin, double* __restr
     double s = 0.0, m = 0.0;
     if (g < n) {
         const int64_t nx = p.nx, ny = p.ny, nz = p.nz, pl = nx * ny;
-        const int64_t i = g % nx, rest = g / nx, j = rest % ny, k = rest / ny;
+        int64_t i, j, k;
+        split3(g, nx, ny, i, j, k);
         const bool ii = i >= 1 && i <= nx - 2, ji = j >= 1 && j <= ny - 2, ki = k >= 1 && k <= nz - 2;
         auto Hn = [&](int64_t o) { return Hin[o] + (double)dl[o]; };
         auto Dv = [&](int64_t o) { return DC == PTD_STEP ? D[o] : 0.0; };
@@ -1646,7 +1672,8 @@ k_grad_accum(KP<double> p, const double* __restrict__ mu, const double* __restri
     const int64_t c = (int64_t)blockIdx.x * kBlock + threadIdx.x;
     if (c >= n) return;
     const int64_t nx = p.nx, ny = p.ny, nz = p.nz;
-    const int64_t i = c % nx, rest = c / nx, j = rest % ny, k = rest / ny;
+    int64_t i, j, k;
+    split3(c, nx, ny, i, j, k);
     const int64_t idx[3] = {i, j, k};
     const int64_t nn[3] = {nx, ny, nz};
     const int64_t st[3] = {1, nx, nx * ny};
