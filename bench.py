@@ -16,6 +16,9 @@ Extra legs (N=1), each in its own object of the JSON line:
    p50 / p99 latency);
  - gradient: dL/dD of the 3D step case at 255^3 over 10 steps by the adjoint, checked by central
    differences on 20 directions;
+ - config_2d: the reference's CPU-runnable case, 2D linear 512^2 over 5 steps (EXACT and FAST, checks
+   every 10 and every nx iterations), with the CPU oracle's time-to-solution on the same case and a
+   bitwise comparison of the EXACT result;
  - cpu_baseline: the CPU oracle timed on the host cores on a bounded sample.
 """
 import argparse
@@ -201,6 +204,62 @@ def cpu_baseline(pkg, n, seconds):
                       f"{el:.1f} s, t_it = {t_it * 1e3:.1f} ms"}
 
 
+CONFIG_2D = "diff_2D_lin"  # 2D linear D=1, lx=ly=10, 5 backward-Euler steps, tol 1e-8 (scripts/diff_2D_lin.jl)
+PUBLISHED_2D_LIN_512 = 2660  # total PT iterations over the 5 steps at 512^2, nout=10 (output/out_diff_2D_lin.txt:5)
+
+
+def config_2d_leg(pkg, n=512):
+    """The reference's CPU-runnable case on the GPU: 2D linear n^2, 5 steps to tol 1e-8 with its IC (the
+    Gaussian's own boundary values stay Dirichlet), checked every 10 iterations (reference parity) and every
+    nx iterations; EXACT arithmetic gives the reference's bits, FAST the rearranged rules."""
+    import numpy as np
+
+    spec = pkg.SCRIPTS[CONFIG_2D]
+    H0 = pkg.initial_condition(spec, n, n)
+    out = {"workload": f"2D linear D=1, {n}^2, 5 steps, tol 1e-8", "published_total_iters": PUBLISHED_2D_LIN_512}
+    for label, arith, nout in (("exact_nout10", 0, 10), ("fast_nout10", 1, 10), ("fast_ncheck_nx", 1, n)):
+        p = pkg.make_params(spec, n, n, arith=arith)
+        p.nout = nout
+        with pkg.Context(p) as ctx:
+            qz = [np.zeros(pkg.field_shape(p, f), order="F") for f in (3, 4)]  # qHx, qHy
+            for rep in range(2):  # the first solve warms the caches and the launch path
+                ctx.set("H", H0)
+                ctx.set("Hold", H0)
+                ctx.set("qHx", qz[0])  # the script's q arrays start at zero (@zeros)
+                ctx.set("qHy", qz[1])
+                t0 = time.perf_counter()
+                iters, errs = ctx.solve(5)
+                el = time.perf_counter() - t0
+            H = ctx.get("H")
+        out[label] = {"s": round(el, 5), "iters": [int(v) for v in iters], "total_iters": int(iters.sum()),
+                      "final_err": float(errs[-1])}
+        if label == "exact_nout10":
+            out["_H_exact"] = H
+    log(f"2D linear {n}^2: " + ", ".join(f"{k} {v['s'] * 1e3:.1f} ms / {v['total_iters']} it"
+                                        for k, v in out.items() if isinstance(v, dict)))
+    return out
+
+
+def cpu_config_2d(pkg, gpu, n=512):
+    """The oracle (OpenMP C restatement of the reference Threads path) on the same 2D case, nout=10: its
+    time-to-solution, iterations, and whether the GPU's EXACT result equals it bit for bit."""
+    import numpy as np
+    from oracle import oracle as O
+
+    case = O.make_case(O.SCRIPTS[CONFIG_2D], n, n)
+    H0 = pkg.initial_condition(pkg.SCRIPTS[CONFIG_2D], n, n)
+    H, Hold = H0.copy(order="F"), H0.copy(order="F")
+    t0 = time.perf_counter()
+    iters, errs, st = O.solve(case, H, Hold)
+    el = time.perf_counter() - t0
+    threads = int(os.environ.get("OMP_NUM_THREADS", os.cpu_count() or 1))
+    res = {"s": round(el, 4), "cores": threads, "iters": [int(v) for v in iters], "total_iters": int(iters.sum())}
+    if gpu is not None and "_H_exact" in gpu:
+        res["gpu_exact_bitwise_equal"] = bool(np.array_equal(gpu["_H_exact"], H))
+        res["gpu_exact_speedup"] = round(el / gpu["exact_nout10"]["s"], 1)
+    return res
+
+
 def main():
     ap = argparse.ArgumentParser()
     ap.add_argument("--gpus", type=int, default=1)
@@ -213,6 +272,7 @@ def main():
                          "exact: reference operation order, bit-identical")
     ap.add_argument("--no-cpu", action="store_true", help="skip the CPU baseline leg")
     ap.add_argument("--no-tts", action="store_true", help="skip the time-to-solution leg")
+    ap.add_argument("--no-2d", action="store_true", help="skip the 2D linear 512^2 leg")
     ap.add_argument("--repeats", type=int, default=5,
                     help="repeats of the K timed iterations for the median / max t_it (0: none)")
     ap.add_argument("--cpu-seconds", type=float, default=15.0)
@@ -380,6 +440,10 @@ def main():
                                         "mixed_vs_fp64_rel_maxnorm": err_m})
         log(f"mixed {tts_m:.3f} s ({tts / tts_m:.2f}x), sweeps {list(miters)}, max-norm vs FP64 {err_m:.2e}")
 
+    cfg2d = None
+    if not args.no_2d:
+        cfg2d = config_2d_leg(pkg)
+
     if args.batch:
         out["batch"] = batch_leg(pkg, args.batch)
 
@@ -389,6 +453,12 @@ def main():
     if not args.no_cpu:
         log(f"CPU baseline (~{args.cpu_seconds:.0f} s)")
         out["cpu_baseline"] = cpu_baseline(pkg, n, args.cpu_seconds)
+        if cfg2d is not None:
+            log("CPU oracle on the 2D linear 512^2 case")
+            cfg2d["cpu_oracle"] = cpu_config_2d(pkg, cfg2d)
+    if cfg2d is not None:
+        cfg2d.pop("_H_exact", None)
+        out["config_2d"] = cfg2d
 
     print(json.dumps(out), flush=True)
     return 0
