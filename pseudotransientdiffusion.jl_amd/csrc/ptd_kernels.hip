@@ -1424,16 +1424,16 @@ hipError_t launch_flux_state(ptd_ctx* c, bool to_p) {
 }
 
 // Start of a correction (one pass over every cell of the float64 iterate Hc): the consistent residual
-// R of the float64 system on the inner cells (residual fluxes from Hc itself, the same expressions as
-// compute_flux_res! + check_res!, scripts/diff_3D/diff_3D_nonlin_perf.jl:44-53 in the context's form),
+// R of the float64 system on the inner cells (residual fluxes from Hc itself: compute_flux_res! +
+// check_res!, scripts/diff_3D/diff_3D_nonlin_perf.jl:44-53, in the context's form and arithmetic),
 // written to ResH, with its per-block (sum r^2, max |r|); the float32 source src = R*dt; float32(Hc) for
 // the correction sweep's aux channel (D = H^3); and delta = 0.
-template <int DC, int FORM>
+template <int DC, int FORM, int ARITH>
 __global__ void __launch_bounds__(kBlock)
 k_outer_residual(const KP<double> p, const double* __restrict__ H, const double* __restrict__ Hold,
                  const double* __restrict__ D, double* __restrict__ resh, float* __restrict__ src,
                  float* __restrict__ hcf, float* __restrict__ dl, double2* __restrict__ partial, int64_t n) {
-    using Ph = Phys<double, DC, FORM, kExact>;
+    using Ph = Phys<double, DC, FORM, ARITH>;
     const int64_t g = (int64_t)blockIdx.x * kBlock + threadIdx.x;
     double s = 0.0, m = 0.0;
     if (g < n) {
@@ -1452,8 +1452,8 @@ k_outer_residual(const KP<double> p, const double* __restrict__ H, const double*
             const double fyn = Ph::face_res(p, Hc, H[g + nx], Dc, Dv(g + nx), p.dy, p._dy);
             const double fzd = Ph::face_res(p, H[g - pl], Hc, Dv(g - pl), Dc, p.dz, p._dz);
             const double fzu = Ph::face_res(p, Hc, H[g + pl], Dc, Dv(g + pl), p.dz, p._dz);
-            const double dv = divergence<double, FORM, 3>(p, fxe - fxw, fyn - fys, fzu - fzd);
-            const double r = residual<double, FORM>(p, Hc, Hold[g], dv);
+            const double dv = Ph::template div<3>(p, fxe - fxw, fyn - fys, fzu - fzd);
+            const double r = Ph::resid(p, Hc, Hold[g], dv);
             resh[(i - 1) + p.mx * ((j - 1) + p.my * (k - 1))] = r;
             src[g] = (float)(r * p.dt);
             s = r * r;
@@ -1470,14 +1470,14 @@ k_outer_residual(const KP<double> p, const double* __restrict__ H, const double*
 // boundary cells), the correction fluxes move from Hc to Hn, Hn goes to the other H buffer, and the
 // residual of the float64 system at Hn gives ResH, the next source R*dt, float32(Hn) and the partials.
 // The next correction's delta = 0 goes to the other delta buffer, so no thread writes what another reads.
-template <int DC, int FORM>
+template <int DC, int FORM, int ARITH>
 __global__ void __launch_bounds__(kBlock)
 k_outer_step(const KP<double> p, const double* __restrict__ Hin, double* __restrict__ Hout,
              const float* __restrict__ dl, float* __restrict__ dl_next, const double* __restrict__ Hold,
              const double* __restrict__ D, double* __restrict__ resh, float* __restrict__ src,
              float* __restrict__ hcf, float* __restrict__ px, float* __restrict__ py, float* __restrict__ pz,
              double3 sc, double3 isc, double2* __restrict__ partial, int64_t n) {
-    using Ph = Phys<double, DC, FORM, kExact>;
+    using Ph = Phys<double, DC, FORM, ARITH>;
     const int64_t g = (int64_t)blockIdx.x * kBlock + threadIdx.x;
     double s = 0.0, m = 0.0;
     if (g < n) {
@@ -1522,8 +1522,8 @@ k_outer_step(const KP<double> p, const double* __restrict__ Hin, double* __restr
             const double fyn = Ph::face_res(p, Hc, Hn(g + nx), Dc, Dv(g + nx), p.dy, p._dy);
             const double fzd = Ph::face_res(p, Hn(g - pl), Hc, Dv(g - pl), Dc, p.dz, p._dz);
             const double fzu = Ph::face_res(p, Hc, Hn(g + pl), Dc, Dv(g + pl), p.dz, p._dz);
-            const double dv = divergence<double, FORM, 3>(p, fxe - fxw, fyn - fys, fzu - fzd);
-            const double r = residual<double, FORM>(p, Hc, Hold[g], dv);
+            const double dv = Ph::template div<3>(p, fxe - fxw, fyn - fys, fzu - fzd);
+            const double r = Ph::resid(p, Hc, Hold[g], dv);
             resh[(i - 1) + p.mx * ((j - 1) + p.my * (k - 1))] = r;
             src[g] = (float)(r * p.dt);
             s = r * r;
@@ -1535,7 +1535,7 @@ k_outer_step(const KP<double> p, const double* __restrict__ Hin, double* __restr
     block_sum_max(s, m, threadIdx.x, kBlock, &partial[blockIdx.x]);
 }
 
-template <int DC, int FORM>
+template <int DC, int FORM, int ARITH>
 static hipError_t outer_step_t(ptd_ctx* c) {
     const KP<double> kp = make_kp<double>(c->p);
     const unsigned gb = grid_for(c->nH);
@@ -1544,7 +1544,7 @@ static hipError_t outer_step_t(ptd_ctx* c) {
     const double3 sc = make_double3(corr_flux_scale(c->p, dc, 0), corr_flux_scale(c->p, dc, 1),
                                     corr_flux_scale(c->p, dc, 2));
     const double3 isc = make_double3(1.0 / sc.x, 1.0 / sc.y, 1.0 / sc.z);
-    hipLaunchKernelGGL((k_outer_step<DC, FORM>), dim3(gb), dim3(kBlock), 0, c->stream, kp,
+    hipLaunchKernelGGL((k_outer_step<DC, FORM, ARITH>), dim3(gb), dim3(kBlock), 0, c->stream, kp,
                        (const double*)c->H[c->hcur], (double*)c->H[1 - c->hcur], (const float*)c->dl[c->dcur],
                        (float*)c->dl[1 - c->dcur], (const double*)c->Hold, (const double*)c->D, (double*)c->resh,
                        (float*)c->src, (float*)c->hcf, (float*)c->pq[0][c->pcur], (float*)c->pq[1][c->pcur],
@@ -1559,21 +1559,28 @@ static hipError_t outer_step_t(ptd_ctx* c) {
     return e;
 }
 
-hipError_t launch_outer_step(ptd_ctx* c) {
+// the residual of the float64 system in the context's arithmetic (EXACT: the reference's check_res!
+// expressions; FAST: the rearranged ones, no divisions)
+template <int ARITH>
+static hipError_t outer_step_a(ptd_ctx* c) {
     const bool perf = c->p.form == PTD_PERF;
     switch (c->p.dcase) {
-        case PTD_LIN: return perf ? outer_step_t<PTD_LIN, PTD_PERF>(c) : outer_step_t<PTD_LIN, PTD_PLAIN>(c);
-        case PTD_STEP: return perf ? outer_step_t<PTD_STEP, PTD_PERF>(c) : outer_step_t<PTD_STEP, PTD_PLAIN>(c);
-        default: return perf ? outer_step_t<PTD_NONLIN, PTD_PERF>(c) : outer_step_t<PTD_NONLIN, PTD_PLAIN>(c);
+        case PTD_LIN: return perf ? outer_step_t<PTD_LIN, PTD_PERF, ARITH>(c) : outer_step_t<PTD_LIN, PTD_PLAIN, ARITH>(c);
+        case PTD_STEP: return perf ? outer_step_t<PTD_STEP, PTD_PERF, ARITH>(c) : outer_step_t<PTD_STEP, PTD_PLAIN, ARITH>(c);
+        default: return perf ? outer_step_t<PTD_NONLIN, PTD_PERF, ARITH>(c) : outer_step_t<PTD_NONLIN, PTD_PLAIN, ARITH>(c);
     }
 }
 
-template <int DC, int FORM>
+hipError_t launch_outer_step(ptd_ctx* c) {
+    return c->p.arith == PTD_ARITH_FAST ? outer_step_a<kFast>(c) : outer_step_a<kExact>(c);
+}
+
+template <int DC, int FORM, int ARITH>
 static hipError_t outer_residual_t(ptd_ctx* c) {
     const KP<double> kp = make_kp<double>(c->p);
     const unsigned gb = grid_for(c->nH);
     if ((int64_t)gb > c->npartial) return hipErrorInvalidValue;
-    hipLaunchKernelGGL((k_outer_residual<DC, FORM>), dim3(gb), dim3(kBlock), 0, c->stream, kp,
+    hipLaunchKernelGGL((k_outer_residual<DC, FORM, ARITH>), dim3(gb), dim3(kBlock), 0, c->stream, kp,
                        (const double*)c->H[c->hcur], (const double*)c->Hold, (const double*)c->D, (double*)c->resh,
                        (float*)c->src, (float*)c->hcf, (float*)c->dl[c->dcur], (double2*)c->partial, c->nH);
     hipLaunchKernelGGL(k_reduce_partials, dim3(1), dim3(kBlock), 0, c->stream, (const double2*)c->partial,
@@ -1581,13 +1588,18 @@ static hipError_t outer_residual_t(ptd_ctx* c) {
     return hipGetLastError();
 }
 
-hipError_t launch_outer_residual(ptd_ctx* c) {
+template <int ARITH>
+static hipError_t outer_residual_a(ptd_ctx* c) {
     const bool perf = c->p.form == PTD_PERF;
     switch (c->p.dcase) {
-        case PTD_LIN: return perf ? outer_residual_t<PTD_LIN, PTD_PERF>(c) : outer_residual_t<PTD_LIN, PTD_PLAIN>(c);
-        case PTD_STEP: return perf ? outer_residual_t<PTD_STEP, PTD_PERF>(c) : outer_residual_t<PTD_STEP, PTD_PLAIN>(c);
-        default: return perf ? outer_residual_t<PTD_NONLIN, PTD_PERF>(c) : outer_residual_t<PTD_NONLIN, PTD_PLAIN>(c);
+        case PTD_LIN: return perf ? outer_residual_t<PTD_LIN, PTD_PERF, ARITH>(c) : outer_residual_t<PTD_LIN, PTD_PLAIN, ARITH>(c);
+        case PTD_STEP: return perf ? outer_residual_t<PTD_STEP, PTD_PERF, ARITH>(c) : outer_residual_t<PTD_STEP, PTD_PLAIN, ARITH>(c);
+        default: return perf ? outer_residual_t<PTD_NONLIN, PTD_PERF, ARITH>(c) : outer_residual_t<PTD_NONLIN, PTD_PLAIN, ARITH>(c);
     }
+}
+
+hipError_t launch_outer_residual(ptd_ctx* c) {
+    return c->p.arith == PTD_ARITH_FAST ? outer_residual_a<kFast>(c) : outer_residual_a<kExact>(c);
 }
 
 // One or two float32 correction iterations on (delta, correction fluxes) with source src = R*dt in
