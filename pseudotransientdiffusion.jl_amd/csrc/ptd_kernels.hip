@@ -716,20 +716,21 @@ struct PlaneOps2 {
 };
 
 // Occupancy (measured at 511^3): two blocks per CU without spills is what pays. float32: 16 rows,
-// operands prefetched one plane ahead, <= 64 VGPRs (8 waves per SIMD). float64 FAST lin / nonlin:
-// 12 rows, two planes ahead, <= 80 VGPRs (6 waves per SIMD). The float64 variants with more live
-// values (checking, step D, EXACT) would spill under that cap and run 16 rows, one block per CU.
+// operands prefetched one plane ahead, <= 64 VGPRs (8 waves per SIMD); its checking variant 12 rows
+// (6 waves per SIMD). float64 lin / nonlin: 12 rows, <= 80 VGPRs (6 waves per SIMD), operands two
+// planes ahead for the FAST sweep and one for the checking and EXACT ones. The float64 variants with
+// more live values (step D, EXACT checking) would spill under that cap and run 16 rows, one block per CU.
 template <typename T, int DC, int ARITH, bool CHECK>
 constexpr bool t2_compact() {
-    return sizeof(T) == 8 && ARITH == kFast && DC != PTD_STEP;
+    return sizeof(T) == 8 && DC != PTD_STEP && (ARITH == kFast || !CHECK);
 }
 template <typename T, int DC, int ARITH, int R, bool CHECK>
 constexpr int t2_waves_per_simd() {
     return sizeof(T) == 4 ? 2 * (R / 4) : (t2_compact<T, DC, ARITH, CHECK>() ? 2 * (R / 4) : 1);
 }
-template <typename T, bool CHECK>
+template <typename T, int ARITH, bool CHECK>
 constexpr int t2_prefetch_depth() {
-    return sizeof(T) == 4 || CHECK ? 1 : 2;
+    return sizeof(T) == 4 || CHECK || ARITH == kExact ? 1 : 2;
 }
 template <typename T, int DC, int ARITH, bool CHECK>
 constexpr int t2_rows() {
@@ -847,7 +848,7 @@ __device__ __forceinline__ void fused3d_t2_tile(const KP<T>& p, const T* __restr
     T q1zL = Ph::template face<false>(p, ldc<T>(bqz, oQz), ldc<T>(bHin, oH), Hc, Dm1, Dc, p.dz, p._dz, p._dz8,
                                       unused);
 
-    constexpr bool kAhead2 = t2_prefetch_depth<T, CHECK>() == 2;
+    constexpr bool kAhead2 = t2_prefetch_depth<T, ARITH, CHECK>() == 2;
     PlaneOps2<T, Ta> A = load_plane(ka);
     PlaneOps2<T, Ta> B;
     if constexpr (kAhead2) B = load_plane(ka + 1 <= nz - 1 ? ka + 1 : nz - 1);
@@ -1834,7 +1835,9 @@ int batch_rows_check(int kind, int dcase, int arith) {
 int batch_rows_nocheck(int kind, int dcase, int arith, int variant) {
     if (kind == kSweep2D2) return kTy2D;
     if (kind == kSweep2D) return variant == 1 ? 8 : 16;
-    return (arith == PTD_ARITH_FAST && dcase != PTD_STEP) ? 12 : 16;
+    if (dcase == PTD_STEP)
+        return arith == PTD_ARITH_FAST ? t2_rows<double, PTD_STEP, kFast, false>() : t2_rows<double, PTD_STEP, kExact, false>();
+    return arith == PTD_ARITH_FAST ? t2_rows<double, PTD_NONLIN, kFast, false>() : t2_rows<double, PTD_NONLIN, kExact, false>();
 }
 
 template <int DC, int ARITH>
