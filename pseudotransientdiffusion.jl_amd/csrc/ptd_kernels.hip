@@ -1467,89 +1467,124 @@ k_outer_residual(const KP<double> p, const double* __restrict__ H, const double*
 }
 
 // Correction boundary in one pass (flux-state move, Hc += delta and k_outer_residual's outputs): with the new
-// iterate Hn = Hc + delta evaluated on the fly at every cell and its 6 neighbours (delta is 0 on the
-// boundary cells), the correction fluxes move from Hc to Hn, Hn goes to the other H buffer, and the
-// residual of the float64 system at Hn gives ResH, the next source R*dt, float32(Hn) and the partials.
-// The next correction's delta = 0 goes to the other delta buffer, so no thread writes what another reads.
+// iterate Hn = Hc + delta evaluated on the fly (delta is 0 on the boundary cells), the correction fluxes
+// move from Hc to Hn, Hn goes to the other H buffer, and the residual of the float64 system at Hn gives
+// ResH, the next source R*dt, float32(Hn) and the partials. The next correction's delta = 0 goes to the
+// other delta buffer, so no thread writes what another reads. A thread owns one (i, j) column of a z chunk
+// and marches along z with the z neighbours in registers (each plane of Hc and delta is read once from
+// HBM; the x / y neighbours come from the same plane through the caches).
 template <int DC, int FORM, int ARITH>
 __global__ void __launch_bounds__(kBlock)
 k_outer_step(const KP<double> p, const double* __restrict__ Hin, double* __restrict__ Hout,
              const float* __restrict__ dl, float* __restrict__ dl_next, const double* __restrict__ Hold,
              const double* __restrict__ D, double* __restrict__ resh, float* __restrict__ src,
              float* __restrict__ hcf, float* __restrict__ px, float* __restrict__ py, float* __restrict__ pz,
-             double3 sc, double3 isc, double2* __restrict__ partial, int64_t n) {
+             double3 sc, double3 isc, double2* __restrict__ partial, int tz) {
     using Ph = Phys<double, DC, FORM, ARITH>;
-    const int64_t g = (int64_t)blockIdx.x * kBlock + threadIdx.x;
+    const int64_t nx = p.nx, ny = p.ny, nz = p.nz, pl = nx * ny;
+    const int64_t nbx = (nx + kBlock - 1) / kBlock;  // blocks per row; blockIdx.x = row block + nbx * j
+    const int64_t i = (int64_t)(blockIdx.x % nbx) * kBlock + threadIdx.x, j = blockIdx.x / nbx;
+    const int64_t k0 = (int64_t)blockIdx.y * tz, k1 = k0 + tz < nz ? k0 + tz : nz;
+    const int64_t part = (int64_t)blockIdx.y * gridDim.x + blockIdx.x;
     double s = 0.0, m = 0.0;
-    if (g < n) {
-        const int64_t nx = p.nx, ny = p.ny, nz = p.nz, pl = nx * ny;
-        int64_t i, j, k;
-        split3(g, nx, ny, i, j, k);
-        const bool ii = i >= 1 && i <= nx - 2, ji = j >= 1 && j <= ny - 2, ki = k >= 1 && k <= nz - 2;
+    if (i < nx) {
+        const bool ii = i >= 1 && i <= nx - 2, ji = j >= 1 && j <= ny - 2;
         auto Hn = [&](int64_t o) { return Hin[o] + (double)dl[o]; };
         auto Dv = [&](int64_t o) { return DC == PTD_STEP ? D[o] : 0.0; };
-        auto gflux = [&](double Ha, double Hb, int64_t a, int64_t b, double ih) {
+        auto gflux = [&](double Ha, double Hb, double Da, double Db, double ih) {
             double coef;
             if constexpr (DC == PTD_NONLIN) {
                 const double hf = 0.5 * (Ha + Hb);
                 coef = hf * hf * hf;
             } else if constexpr (DC == PTD_STEP) {
-                coef = (D[a] + D[b]) * 0.5;
+                coef = (Da + Db) * 0.5;
             } else {
                 coef = p.d0;
             }
             return coef * (ih * (Hb - Ha));
         };
-        auto move = [&](float* pf, int64_t f, int64_t a, double ih, double sa, double isa) {
-            const double go = gflux(Hin[a], Hin[g], a, g, ih);
-            const double gn = gflux(Hn(a), Hn(g), a, g, ih);
+        // move of the face (a | this cell): old Hc and new Hn at both cells
+        auto move = [&](float* pf, int64_t f, double Hoa, double Hob, double Hna, double Hnb, double Da, double Db,
+                        double ih, double sa, double isa) {
+            const double go = gflux(Hoa, Hob, Da, Db, ih);
+            const double gn = gflux(Hna, Hnb, Da, Db, ih);
             pf[f] = (float)(((double)pf[f] * isa - go + gn) * sa);
         };
-        if (i >= 1 && ji && ki)
-            move(px, (i - 1) + (nx - 1) * ((j - 1) + (ny - 2) * (k - 1)), g - 1, p._dx, sc.x, isc.x);
-        if (j >= 1 && ii && ki)
-            move(py, (i - 1) + (nx - 2) * ((j - 1) + (ny - 1) * (k - 1)), g - nx, p._dy, sc.y, isc.y);
-        if (k >= 1 && ii && ji)
-            move(pz, (i - 1) + (nx - 2) * ((j - 1) + (ny - 2) * (k - 1)), g - pl, p._dz, sc.z, isc.z);
-        const double Hc = Hn(g);
-        if (hcf) hcf[g] = (float)Hc;
-        dl_next[g] = 0.0f;
-        if (ii && ji && ki) {
-            Hout[g] = Hc;
-            const double Dc = Dv(g);
-            const double fxw = Ph::face_res(p, Hn(g - 1), Hc, Dv(g - 1), Dc, p.dx, p._dx);
-            const double fxe = Ph::face_res(p, Hc, Hn(g + 1), Dc, Dv(g + 1), p.dx, p._dx);
-            const double fys = Ph::face_res(p, Hn(g - nx), Hc, Dv(g - nx), Dc, p.dy, p._dy);
-            const double fyn = Ph::face_res(p, Hc, Hn(g + nx), Dc, Dv(g + nx), p.dy, p._dy);
-            const double fzd = Ph::face_res(p, Hn(g - pl), Hc, Dv(g - pl), Dc, p.dz, p._dz);
-            const double fzu = Ph::face_res(p, Hc, Hn(g + pl), Dc, Dv(g + pl), p.dz, p._dz);
-            const double dv = Ph::template div<3>(p, fxe - fxw, fyn - fys, fzu - fzd);
-            const double r = Ph::resid(p, Hc, Hold[g], dv);
-            resh[(i - 1) + p.mx * ((j - 1) + p.my * (k - 1))] = r;
-            src[g] = (float)(r * p.dt);
-            s = r * r;
-            m = fabs(r);
-        } else {
-            src[g] = 0.0f;
+        const int64_t col = i + nx * j;
+        int64_t g = col + pl * k0;
+        // z window: old Hc, new Hn and D of planes k-1 and k
+        double Hom = 0.0, Hnm = 0.0, Dm = 0.0;
+        if (k0 >= 1) {
+            Hom = Hin[g - pl];
+            Hnm = Hom + (double)dl[g - pl];
+            Dm = Dv(g - pl);
+        }
+        double Hoc = Hin[g];
+        double Hnc = Hoc + (double)dl[g];
+        double Dc = Dv(g);
+        for (int64_t k = k0; k < k1; ++k, g += pl) {
+            const bool ki = k >= 1 && k <= nz - 2;
+            double Hop = 0.0, Hnp = 0.0, Dp = 0.0;  // plane k+1
+            if (k + 1 < nz) {
+                Hop = Hin[g + pl];
+                Hnp = Hop + (double)dl[g + pl];
+                Dp = Dv(g + pl);
+            }
+            if (i >= 1 && ji && ki)
+                move(px, (i - 1) + (nx - 1) * ((j - 1) + (ny - 2) * (k - 1)), Hin[g - 1], Hoc, Hn(g - 1), Hnc,
+                     Dv(g - 1), Dc, p._dx, sc.x, isc.x);
+            if (j >= 1 && ii && ki)
+                move(py, (i - 1) + (nx - 2) * ((j - 1) + (ny - 1) * (k - 1)), Hin[g - nx], Hoc, Hn(g - nx), Hnc,
+                     Dv(g - nx), Dc, p._dy, sc.y, isc.y);
+            if (k >= 1 && ii && ji)
+                move(pz, (i - 1) + (nx - 2) * ((j - 1) + (ny - 2) * (k - 1)), Hom, Hoc, Hnm, Hnc, Dm, Dc, p._dz,
+                     sc.z, isc.z);
+            if (hcf) hcf[g] = (float)Hnc;
+            dl_next[g] = 0.0f;
+            if (ii && ji && ki) {
+                Hout[g] = Hnc;
+                const double fxw = Ph::face_res(p, Hn(g - 1), Hnc, Dv(g - 1), Dc, p.dx, p._dx);
+                const double fxe = Ph::face_res(p, Hnc, Hn(g + 1), Dc, Dv(g + 1), p.dx, p._dx);
+                const double fys = Ph::face_res(p, Hn(g - nx), Hnc, Dv(g - nx), Dc, p.dy, p._dy);
+                const double fyn = Ph::face_res(p, Hnc, Hn(g + nx), Dc, Dv(g + nx), p.dy, p._dy);
+                const double fzd = Ph::face_res(p, Hnm, Hnc, Dm, Dc, p.dz, p._dz);
+                const double fzu = Ph::face_res(p, Hnc, Hnp, Dc, Dp, p.dz, p._dz);
+                const double dv = Ph::template div<3>(p, fxe - fxw, fyn - fys, fzu - fzd);
+                const double r = Ph::resid(p, Hnc, Hold[g], dv);
+                resh[(i - 1) + p.mx * ((j - 1) + p.my * (k - 1))] = r;
+                src[g] = (float)(r * p.dt);
+                s += r * r;
+                m = nanmax(fabs(r), m);
+            } else {
+                src[g] = 0.0f;
+            }
+            Hom = Hoc;
+            Hnm = Hnc;
+            Dm = Dc;
+            Hoc = Hop;
+            Hnc = Hnp;
+            Dc = Dp;
         }
     }
-    block_sum_max(s, m, threadIdx.x, kBlock, &partial[blockIdx.x]);
+    block_sum_max(s, m, threadIdx.x, kBlock, &partial[part]);
 }
 
 template <int DC, int FORM, int ARITH>
 static hipError_t outer_step_t(ptd_ctx* c) {
     const KP<double> kp = make_kp<double>(c->p);
-    const unsigned gb = grid_for(c->nH);
-    if ((int64_t)gb > c->npartial) return hipErrorInvalidValue;
+    constexpr int kTzStep = 32;  // planes per thread
+    const dim3 grid((unsigned)((c->p.nx + kBlock - 1) / kBlock * c->p.ny), (unsigned)((c->p.nz + kTzStep - 1) / kTzStep));
+    const int64_t gb = (int64_t)grid.x * grid.y * grid.z;
+    if (gb > c->npartial) return hipErrorInvalidValue;
     const int dc = c->p.dcase;
     const double3 sc = make_double3(corr_flux_scale(c->p, dc, 0), corr_flux_scale(c->p, dc, 1),
                                     corr_flux_scale(c->p, dc, 2));
     const double3 isc = make_double3(1.0 / sc.x, 1.0 / sc.y, 1.0 / sc.z);
-    hipLaunchKernelGGL((k_outer_step<DC, FORM, ARITH>), dim3(gb), dim3(kBlock), 0, c->stream, kp,
+    hipLaunchKernelGGL((k_outer_step<DC, FORM, ARITH>), grid, dim3(kBlock), 0, c->stream, kp,
                        (const double*)c->H[c->hcur], (double*)c->H[1 - c->hcur], (const float*)c->dl[c->dcur],
                        (float*)c->dl[1 - c->dcur], (const double*)c->Hold, (const double*)c->D, (double*)c->resh,
                        (float*)c->src, (float*)c->hcf, (float*)c->pq[0][c->pcur], (float*)c->pq[1][c->pcur],
-                       (float*)c->pq[2][c->pcur], sc, isc, (double2*)c->partial, c->nH);
+                       (float*)c->pq[2][c->pcur], sc, isc, (double2*)c->partial, kTzStep);
     hipLaunchKernelGGL(k_reduce_partials, dim3(1), dim3(kBlock), 0, c->stream, (const double2*)c->partial,
                        (int64_t)gb, c->dred);
     hipError_t e = hipGetLastError();
