@@ -148,7 +148,8 @@ int member_tiles(const ptd_params& p, int kind, int rows, int tz) {
     return (int)(ntx * nty);
 }
 
-int check_tz(const ptd_params& p) { return (int)std::min<int64_t>(16, p.nz - 2); }
+// the single-problem sweeps' z chunk depth (fused_tz): the checking launch must use their tiling
+int check_tz(const ptd_params& p) { return (int)std::min<int64_t>(32, p.nz - 2); }
 
 int upload_table(ptd_batch* b, const std::vector<BatchTile>& t, BatchTile** d) {
     int st = balloc(b, (void**)d, t.size() * sizeof(BatchTile));
