@@ -187,7 +187,9 @@ int32_t ptd_solve(ptd_ctx* ctx, int32_t nt, int64_t* iters, double* errs);
  *   grad_d   = dL/dD per cell (nx*ny*nz; in the lin case the sensitivity to a per-cell D at d0),
  *   grad_h0  = dL/dH0 per cell (inner cells: mu_1/dt; boundary cells: their Dirichlet coupling),
  *   iters_fwd[nt], iters_adj[nt] = PT iterations of each forward / adjoint solve.
- * Any output may be NULL. The context ends as after ptd_solve(nt). float64, PTD_LIN or PTD_STEP. */
+ * Any output may be NULL. On success the context ends as after ptd_solve(nt); after an error (NaN,
+ * allocation) its fields are unspecified and should be set again. Allocates (nt + 4) grid-sized
+ * float64 buffers and a copy of the fluxes for the call (PTD_ERR_NOMEM if they do not fit). float64, PTD_LIN or PTD_STEP. */
 int32_t ptd_grad_solve(ptd_ctx* ctx, int32_t nt, const double* h_obs, int64_t count, double* loss,
                        double* grad_d, double* grad_h0, int64_t* iters_fwd, int64_t* iters_adj);
 

@@ -105,6 +105,7 @@ int dev_alloc(ptd_ctx* c, void** ptr, int64_t n, size_t esz) {
     hipError_t e = hipMalloc(ptr, bytes);
     if (e != hipSuccess) {
         *ptr = nullptr;
+        (void)hipGetLastError();  // clear it: later launch checks read the thread's last error
         return fail(PTD_ERR_NOMEM, "hipMalloc of " + std::to_string(bytes) + " bytes failed: " + hipGetErrorString(e));
     }
     e = hipMemsetAsync(*ptr, 0, bytes, c->stream);

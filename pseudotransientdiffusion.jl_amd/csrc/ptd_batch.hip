@@ -100,6 +100,7 @@ int balloc(ptd_batch* b, void** ptr, size_t bytes) {
     hipError_t e = hipMalloc(ptr, bytes);
     if (e != hipSuccess) {
         *ptr = nullptr;
+        (void)hipGetLastError();  // clear it: later launch checks read the thread's last error
         return ptd::set_error(PTD_ERR_NOMEM, "hipMalloc of " + std::to_string(bytes) + " bytes failed");
     }
     b->allocs.push_back(*ptr);
@@ -135,8 +136,7 @@ void free_batch(ptd_batch* b) {
     delete b;
 }
 
-// tiles of one member for a launch with `rows` rows per block and z chunk depth tz (3D)
-// tiles of one member's launch: rows per block (two-level 2D: rows per tile chunk), z chunk depth tz
+// tiles of one member's launch: rows per block (two-level 2D: rows per tile chunk), z chunk depth tz (3D)
 int member_tiles(const ptd_params& p, int kind, int rows, int tz) {
     if (kind == ptd::kSweep2D2) return (int)(((p.nx - 2 + 59) / 60) * ((p.ny - 2 + rows - 1) / rows));
     if (p.ndim == 3) {

@@ -33,6 +33,17 @@ namespace {
         }                                                                                          \
     } while (0)
 
+#define PTD_ALLOCG(ptr, bytes)                                                                     \
+    do {                                                                                           \
+        const size_t b_ = (bytes);                                                                 \
+        if (hipMalloc((void**)(ptr), b_) != hipSuccess) {                                          \
+            *(ptr) = nullptr;                                                                      \
+            (void)hipGetLastError(); /* later launch checks read the thread's last error */        \
+            status = ptd::set_error(PTD_ERR_NOMEM, "hipMalloc of " + std::to_string(b_) + " bytes failed"); \
+            goto done;                                                                             \
+        }                                                                                          \
+    } while (0)
+
 }  // namespace
 
 extern "C" int32_t ptd_grad_solve(ptd_ctx* c, int32_t nt, const double* h_obs, int64_t count, double* loss,
@@ -50,12 +61,12 @@ extern "C" int32_t ptd_grad_solve(ptd_ctx* c, int32_t nt, const double* h_obs, i
     double* qbak[3] = {nullptr, nullptr, nullptr};
     double red[2] = {0, 0};
     {
-        PTD_HIPG(hipMalloc((void**)&ckpt, (size_t)(nt + 1) * nH * 8));
-        PTD_HIPG(hipMalloc((void**)&hobs, nH * 8));
-        PTD_HIPG(hipMalloc((void**)&gD, nH * 8));
-        PTD_HIPG(hipMalloc((void**)&gH0, nH * 8));
+        PTD_ALLOCG(&ckpt, (size_t)(nt + 1) * nH * 8);
+        PTD_ALLOCG(&hobs, nH * 8);
+        PTD_ALLOCG(&gD, nH * 8);
+        PTD_ALLOCG(&gH0, nH * 8);
         for (int a = 0; a < 3; ++a)
-            if (c->nQ[a] > 0) PTD_HIPG(hipMalloc((void**)&qbak[a], (size_t)c->nQ[a] * 8));
+            if (c->nQ[a] > 0) PTD_ALLOCG(&qbak[a], (size_t)c->nQ[a] * 8);
         PTD_HIPG(hipMemsetAsync(gD, 0, nH * 8, c->stream));
         PTD_HIPG(hipMemsetAsync(gH0, 0, nH * 8, c->stream));
         PTD_HIPG(hipMemcpyAsync(hobs, h_obs, nH * 8, hipMemcpyHostToDevice, c->stream));
