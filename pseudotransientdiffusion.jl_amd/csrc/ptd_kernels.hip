@@ -1504,7 +1504,6 @@ k_outer_step(const KP<double> p, const double* __restrict__ Hin, double* __restr
     double s = 0.0, m = 0.0;
     if (i < nx) {
         const bool ii = i >= 1 && i <= nx - 2, ji = j >= 1 && j <= ny - 2;
-        auto Hn = [&](int64_t o) { return Hin[o] + (double)dl[o]; };
         auto Dv = [&](int64_t o) { return DC == PTD_STEP ? D[o] : 0.0; };
         auto gflux = [&](double Ha, double Hb, double Da, double Db, double ih) {
             double coef;
@@ -1518,15 +1517,23 @@ k_outer_step(const KP<double> p, const double* __restrict__ Hin, double* __restr
             }
             return coef * (ih * (Hb - Ha));
         };
-        // move of the face (a | this cell): old Hc and new Hn at both cells
-        auto move = [&](float* pf, int64_t f, double Hoa, double Hob, double Hna, double Hnb, double Da, double Db,
-                        double ih, double sa, double isa) {
+        // move of the face (a | this cell) with the old flux state pv: old Hc and new Hn at both cells
+        auto moved = [&](float pv, double Hoa, double Hob, double Hna, double Hnb, double Da, double Db, double ih,
+                         double sa, double isa) {
             const double go = gflux(Hoa, Hob, Da, Db, ih);
             const double gn = gflux(Hna, Hnb, Da, Db, ih);
-            pf[f] = (float)(((double)pf[f] * isa - go + gn) * sa);
+            return (float)(((double)pv * isa - go + gn) * sa);
         };
         const int64_t col = i + nx * j;
         int64_t g = col + pl * k0;
+        // flux / ResH offsets of plane k (face arrays and ResH are indexed from the first inner plane; only
+        // used where the face / cell exists), advanced by one plane of each array per step
+        const int64_t sx = (nx - 1) * (ny - 2), sy = (nx - 2) * (ny - 1), sz = (nx - 2) * (ny - 2);
+        int64_t ox = (i - 1) + (nx - 1) * (j - 1) + sx * (k0 - 1);
+        int64_t oy = (i - 1) + (nx - 2) * (j - 1) + sy * (k0 - 1);
+        int64_t oz = (i - 1) + (nx - 2) * (j - 1) + sz * (k0 - 1);  // also ResH: (nx-2, ny-2, nz-2)
+        // x / y neighbour columns, clamped to this column on the boundary (values then unused)
+        const int64_t dw = i >= 1 ? 1 : 0, de = i + 1 < nx ? 1 : 0, ds = j >= 1 ? nx : 0, dn = j + 1 < ny ? nx : 0;
         // z window: old Hc, new Hn and D of planes k-1 and k
         double Hom = 0.0, Hnm = 0.0, Dm = 0.0;
         if (k0 >= 1) {
@@ -1537,36 +1544,39 @@ k_outer_step(const KP<double> p, const double* __restrict__ Hin, double* __restr
         double Hoc = Hin[g];
         double Hnc = Hoc + (double)dl[g];
         double Dc = Dv(g);
-        for (int64_t k = k0; k < k1; ++k, g += pl) {
+        // Every load of a plane step is unconditional, from an in-range address, and issued before the
+        // arithmetic, so the step waits for its loads once.
+        for (int64_t k = k0; k < k1; ++k, g += pl, ox += sx, oy += sy, oz += sz) {
             const bool ki = k >= 1 && k <= nz - 2;
-            double Hop = 0.0, Hnp = 0.0, Dp = 0.0;  // plane k+1
-            if (k + 1 < nz) {
-                Hop = Hin[g + pl];
-                Hnp = Hop + (double)dl[g + pl];
-                Dp = Dv(g + pl);
-            }
-            if (i >= 1 && ji && ki)
-                move(px, (i - 1) + (nx - 1) * ((j - 1) + (ny - 2) * (k - 1)), Hin[g - 1], Hoc, Hn(g - 1), Hnc,
-                     Dv(g - 1), Dc, p._dx, sc.x, isc.x);
-            if (j >= 1 && ii && ki)
-                move(py, (i - 1) + (nx - 2) * ((j - 1) + (ny - 1) * (k - 1)), Hin[g - nx], Hoc, Hn(g - nx), Hnc,
-                     Dv(g - nx), Dc, p._dy, sc.y, isc.y);
-            if (k >= 1 && ii && ji)
-                move(pz, (i - 1) + (nx - 2) * ((j - 1) + (ny - 2) * (k - 1)), Hom, Hoc, Hnm, Hnc, Dm, Dc, p._dz,
-                     sc.z, isc.z);
+            const bool vx = i >= 1 && ji && ki, vy = j >= 1 && ii && ki, vz = k >= 1 && ii && ji;
+            const bool inner = ii && ji && ki;
+            const int64_t gp = k + 1 < nz ? g + pl : g;  // plane k+1 (unused past the last plane)
+            const double Hop = Hin[gp];
+            const float dlp = dl[gp];
+            const double Dp = Dv(gp);
+            const double Hw = Hin[g - dw], He = Hin[g + de], Hs = Hin[g - ds], Hn_ = Hin[g + dn];
+            const float dlw = dl[g - dw], dle = dl[g + de], dls = dl[g - ds], dln = dl[g + dn];
+            const double Dw = Dv(g - dw), De = Dv(g + de), Ds = Dv(g - ds), Dn = Dv(g + dn);
+            const float pxv = px[vx ? ox : 0], pyv = py[vy ? oy : 0], pzv = pz[vz ? oz : 0];
+            const double Ho = Hold[g];
+            const double Hnp = Hop + (double)dlp;
+            const double Hnw = Hw + (double)dlw, Hne = He + (double)dle, Hns = Hs + (double)dls, Hnn = Hn_ + (double)dln;
+            if (vx) px[ox] = moved(pxv, Hw, Hoc, Hnw, Hnc, Dw, Dc, p._dx, sc.x, isc.x);
+            if (vy) py[oy] = moved(pyv, Hs, Hoc, Hns, Hnc, Ds, Dc, p._dy, sc.y, isc.y);
+            if (vz) pz[oz] = moved(pzv, Hom, Hoc, Hnm, Hnc, Dm, Dc, p._dz, sc.z, isc.z);
             if (hcf) hcf[g] = (float)Hnc;
             dl_next[g] = 0.0f;
-            if (ii && ji && ki) {
+            if (inner) {
                 Hout[g] = Hnc;
-                const double fxw = Ph::face_res(p, Hn(g - 1), Hnc, Dv(g - 1), Dc, p.dx, p._dx);
-                const double fxe = Ph::face_res(p, Hnc, Hn(g + 1), Dc, Dv(g + 1), p.dx, p._dx);
-                const double fys = Ph::face_res(p, Hn(g - nx), Hnc, Dv(g - nx), Dc, p.dy, p._dy);
-                const double fyn = Ph::face_res(p, Hnc, Hn(g + nx), Dc, Dv(g + nx), p.dy, p._dy);
+                const double fxw = Ph::face_res(p, Hnw, Hnc, Dw, Dc, p.dx, p._dx);
+                const double fxe = Ph::face_res(p, Hnc, Hne, Dc, De, p.dx, p._dx);
+                const double fys = Ph::face_res(p, Hns, Hnc, Ds, Dc, p.dy, p._dy);
+                const double fyn = Ph::face_res(p, Hnc, Hnn, Dc, Dn, p.dy, p._dy);
                 const double fzd = Ph::face_res(p, Hnm, Hnc, Dm, Dc, p.dz, p._dz);
                 const double fzu = Ph::face_res(p, Hnc, Hnp, Dc, Dp, p.dz, p._dz);
                 const double dv = Ph::template div<3>(p, fxe - fxw, fyn - fys, fzu - fzd);
-                const double r = Ph::resid(p, Hnc, Hold[g], dv);
-                resh[(i - 1) + p.mx * ((j - 1) + p.my * (k - 1))] = r;
+                const double r = Ph::resid(p, Hnc, Ho, dv);
+                resh[oz] = r;
                 src[g] = (float)(r * p.dt);
                 s += r * r;
                 m = nanmax(fabs(r), m);
