@@ -11,7 +11,9 @@
 //    controller kernel applies the reference loop's rules per member (scripts/diff_3D/
 //    diff_3D_nonlin_multixpu.jl:104-118) and members that are done drop out of later launches;
 //  - one check block (all groups on their own streams, then reduction, controller and Hold .= H) is
-//    captured once as a HIP graph and replayed until no member is running;
+//    captured once as a HIP graph and replayed until no member is running; groups of 2D grids wider
+//    than the 3D ones run several check blocks per replay (group_sub_blocks), as they need
+//    proportionally more PT iterations;
 //  - the tiling of the non-checking launches (z chunk depth in 3D, rows per block in 2D) is chosen
 //    online: the first replays try each candidate graph (each replay is real progress, results do not
 //    depend on the tiling) and the fastest is kept.
@@ -49,7 +51,27 @@ struct Group {
     int n_nc[kNumCandidates] = {};
     BatchTile* d_t2 = nullptr;  // single-level 2D groups: two-level tiles of their pair launches
     int n_t2 = 0;
+    int sub = 1;                // check blocks per replay
 };
+
+// Check blocks per replay of each group. Every replay advances each running member by the same number
+// of check blocks, and PT iteration counts grow with the grid's extent (the accelerated iteration needs
+// O(n) iterations per step in 1D, 2D and 3D alike), so 2D members on grids wider than the 3D ones would
+// need several times more replays than the 3D members, and would finish alone, on a GPU they cannot
+// fill. Such groups run floor(n_group / n_3D) check blocks per replay (at most 8), each closed by its own
+// controller, and finish in about as many replays as the 3D members.
+constexpr int kMaxSubBlocks = 8;
+int group_sub_blocks(const std::vector<ptd_params>& params, const std::vector<int32_t>& members) {
+    int64_t n3 = 0, ng = 0;
+    auto extent = [](const ptd_params& p) {
+        return std::max<int64_t>(p.nx, std::max<int64_t>(p.ny, p.ndim == 3 ? p.nz : 0));
+    };
+    for (const auto& p : params)
+        if (p.ndim == 3) n3 = std::max(n3, extent(p));
+    for (int32_t m : members) ng = std::max(ng, extent(params[(size_t)m]));
+    if (n3 == 0) return 1;
+    return (int)std::min<int64_t>(kMaxSubBlocks, std::max<int64_t>(1, ng / n3));
+}
 
 // Launches per check block of nout iterations: two-level sweeps take two iterations per launch. The
 // single-level 2D groups (grids too small for ptd_solve to pick the two-level 2D sweep) still run
@@ -159,28 +181,38 @@ int upload_table(ptd_batch* b, const std::vector<BatchTile>& t, BatchTile** d) {
     return PTD_OK;
 }
 
-// one check block: every group's nout iterations on its own stream, then reduction, controller,
+// one replay: every group's check blocks of nout iterations on its own stream (each but the last closed
+// by the group's own controller and Hold .= H), then reduction, the controller of every member,
 // Hold .= H and the running-member count to the host
 int enqueue_block(ptd_batch* b, int cand) {
     PTD_HIPB(hipEventRecord(b->ev_fork, b->stream));
     for (size_t gi = 0; gi < b->groups.size(); ++gi) {
         Group& g = b->groups[gi];
+        const int nm = (int)g.members.size();
         PTD_HIPB(hipStreamWaitEvent(g.stream, b->ev_fork, 0));
         const int lpb = launches_per_block(g.kind, b->nout);
-        for (int li = 0; li < lpb; ++li) {
-            const bool check = li == lpb - 1;
-            if (g.kind == ptd::kSweep2D && li < lpb - 2) {  // a two-level pair
-                PTD_HIPB(ptd::launch_batch_sweep(g.stream, ptd::kSweep2D2, g.dcase, g.arith,
-                                                 ptd::batch_rows_nocheck(ptd::kSweep2D2, g.dcase, g.arith, 0), false,
-                                                 b->dm, g.d_t2, g.n_t2, li));
-                continue;
+        for (int sb = 0; sb < g.sub; ++sb) {
+            if (sb > 0) {
+                PTD_HIPB(ptd::launch_batch_control_list(g.stream, b->dm, g.d_members, nm, b->d_iters, b->d_errs,
+                                                        b->nt_cap));
+                PTD_HIPB(ptd::launch_batch_assign_list(g.stream, b->dm, g.d_members, nm));
             }
-            const int rows = check ? ptd::batch_rows_check(g.kind, g.dcase, g.arith)
-                                   : ptd::batch_rows_nocheck(g.kind, g.dcase, g.arith, kCandidates[cand].rows2d);
-            PTD_HIPB(ptd::launch_batch_sweep(g.stream, g.kind, g.dcase, g.arith, rows, check, b->dm,
-                                             check ? g.d_check : g.d_nc[cand], check ? g.n_check : g.n_nc[cand], li));
+            for (int li = 0; li < lpb; ++li) {
+                const bool check = li == lpb - 1;
+                if (g.kind == ptd::kSweep2D && li < lpb - 2) {  // a two-level pair
+                    PTD_HIPB(ptd::launch_batch_sweep(g.stream, ptd::kSweep2D2, g.dcase, g.arith,
+                                                     ptd::batch_rows_nocheck(ptd::kSweep2D2, g.dcase, g.arith, 0),
+                                                     false, b->dm, g.d_t2, g.n_t2, li));
+                    continue;
+                }
+                const int rows = check ? ptd::batch_rows_check(g.kind, g.dcase, g.arith)
+                                       : ptd::batch_rows_nocheck(g.kind, g.dcase, g.arith, kCandidates[cand].rows2d);
+                PTD_HIPB(ptd::launch_batch_sweep(g.stream, g.kind, g.dcase, g.arith, rows, check, b->dm,
+                                                 check ? g.d_check : g.d_nc[cand], check ? g.n_check : g.n_nc[cand],
+                                                 li));
+            }
+            PTD_HIPB(ptd::launch_batch_reduce(g.stream, b->dm, g.d_members, nm));
         }
-        PTD_HIPB(ptd::launch_batch_reduce(g.stream, b->dm, g.d_members, (int)g.members.size()));
         PTD_HIPB(hipEventRecord(b->ev_join[gi], g.stream));
         PTD_HIPB(hipStreamWaitEvent(b->stream, b->ev_join[gi], 0));
     }
@@ -321,6 +353,7 @@ int32_t ptd_batch_create(const ptd_params* params, int32_t n, int32_t device, pt
         }
         g->members.push_back(m);
     }
+    for (auto& g : b->groups) g.sub = group_sub_blocks(b->params, g.members);
     for (auto& g : b->groups) {
         if (hipStreamCreate(&g.stream) != hipSuccess) return fail_free(ptd::set_error(PTD_ERR_HIP, "stream creation failed"));
         hipEvent_t e;

@@ -556,7 +556,10 @@ int batch_rows_check(int kind, int dcase, int arith);                 // rows of
 hipError_t launch_batch_reduce(hipStream_t s, BatchMember* mem, const int32_t* members, int n);
 hipError_t launch_batch_control(hipStream_t s, BatchMember* mem, int n, int64_t* iters, double* errs,
                                 int32_t* active_count, int nt_cap);
+hipError_t launch_batch_control_list(hipStream_t s, BatchMember* mem, const int32_t* members, int n, int64_t* iters,
+                                     double* errs, int nt_cap);
 hipError_t launch_batch_assign(hipStream_t s, BatchMember* mem, int n);
+hipError_t launch_batch_assign_list(hipStream_t s, BatchMember* mem, const int32_t* members, int n);
 hipError_t launch_batch_clock(hipStream_t s, uint64_t* t);
 
 }  // namespace ptd

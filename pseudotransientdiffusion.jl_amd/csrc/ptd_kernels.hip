@@ -1973,46 +1973,48 @@ hipError_t launch_batch_reduce(hipStream_t s, BatchMember* mem, const int32_t* m
     return hipGetLastError();
 }
 
-// The PT loop of every member after one check block of nout iterations
+// The PT loop of one member after one check block of nout iterations
 // (scripts/diff_3D/diff_3D_nonlin_multixpu.jl:104-118): iter += nout; err from the check; the step
 // ends when !(err > tol) or iter >= itmax (then Hold .= H, next step); the member is done after nt
-// steps or on NaN. One thread per member; the number of members still running goes to *active_count.
+// steps or on NaN. Returns whether the member is still running.
+__device__ __forceinline__ int control_member(BatchMember& M, int t, int64_t* __restrict__ iters,
+                                              double* __restrict__ errs, int nt_cap) {
+    M.assign = 0;
+    if (M.active) {
+        M.iter += M.nout;
+        M.hpar = (M.hpar + M.lpb) & 1;
+        M.qpar = (M.qpar + M.lpb) & 1;
+        const double rms = sqrt(M.red.x) / sqrt(M.norm_len);
+        const double err = M.stop_norm == PTD_NORM_MAX ? M.red.y : rms;
+        if (!(err > M.tol) || M.iter >= M.itmax) {
+            if (M.step < nt_cap) {
+                iters[(int64_t)t * nt_cap + M.step] = M.iter;
+                errs[(int64_t)t * nt_cap + M.step] = err;
+            }
+            M.step += 1;
+            M.iter = 0;
+            if (isnan(err)) {
+                M.status = PTD_ERR_NAN;
+                M.active = 0;
+                M.t_done = (uint64_t)wall_clock64();
+            } else {
+                M.assign = 1;
+                if (M.step >= M.nt) {
+                    M.active = 0;
+                    M.t_done = (uint64_t)wall_clock64();
+                }
+            }
+        }
+    }
+    return M.active;
+}
+
+// Every member, one thread each; the number of members still running goes to *active_count.
 __global__ void __launch_bounds__(kBatchMaxMembers)
 k_batch_control(BatchMember* __restrict__ mem, int n, int64_t* __restrict__ iters, double* __restrict__ errs,
                 int32_t* __restrict__ active_count, int nt_cap) {
     const int t = threadIdx.x;
-    int running = 0;
-    if (t < n) {
-        BatchMember& M = mem[t];
-        M.assign = 0;
-        if (M.active) {
-            M.iter += M.nout;
-            M.hpar = (M.hpar + M.lpb) & 1;
-            M.qpar = (M.qpar + M.lpb) & 1;
-            const double rms = sqrt(M.red.x) / sqrt(M.norm_len);
-            const double err = M.stop_norm == PTD_NORM_MAX ? M.red.y : rms;
-            if (!(err > M.tol) || M.iter >= M.itmax) {
-                if (M.step < nt_cap) {
-                    iters[(int64_t)t * nt_cap + M.step] = M.iter;
-                    errs[(int64_t)t * nt_cap + M.step] = err;
-                }
-                M.step += 1;
-                M.iter = 0;
-                if (isnan(err)) {
-                    M.status = PTD_ERR_NAN;
-                    M.active = 0;
-                    M.t_done = (uint64_t)wall_clock64();
-                } else {
-                    M.assign = 1;
-                    if (M.step >= M.nt) {
-                        M.active = 0;
-                        M.t_done = (uint64_t)wall_clock64();
-                    }
-                }
-            }
-        }
-        running = M.active;
-    }
+    const int running = t < n ? control_member(mem[t], t, iters, errs, nt_cap) : 0;
     const int cnt = __syncthreads_count(running);
     if (t == 0) *active_count = cnt;
 }
@@ -2025,10 +2027,30 @@ hipError_t launch_batch_control(hipStream_t s, BatchMember* mem, int n, int64_t*
     return hipGetLastError();
 }
 
-// Hold .= H for the members whose step just ended (kBatchAssignBlocks blocks per member)
+// The members of one group (a check block inside a replay: the group runs further check blocks before
+// the replay's closing controller)
+__global__ void __launch_bounds__(kBatchMaxMembers)
+k_batch_control_list(BatchMember* __restrict__ mem, const int32_t* __restrict__ members, int n,
+                     int64_t* __restrict__ iters, double* __restrict__ errs, int nt_cap) {
+    const int t = threadIdx.x;
+    if (t < n) (void)control_member(mem[members[t]], members[t], iters, errs, nt_cap);
+}
+
+hipError_t launch_batch_control_list(hipStream_t s, BatchMember* mem, const int32_t* members, int n, int64_t* iters,
+                                     double* errs, int nt_cap) {
+    if (n <= 0 || n > kBatchMaxMembers) return hipErrorInvalidValue;
+    hipLaunchKernelGGL(k_batch_control_list, dim3(1), dim3(kBatchMaxMembers), 0, s, mem, members, n, iters, errs,
+                       nt_cap);
+    return hipGetLastError();
+}
+
+// Hold .= H for the members whose step just ended (kBatchAssignBlocks blocks per member; `members`
+// lists them, or null for members 0..n-1)
 constexpr int kBatchAssignBlocks = 64;
-__global__ void __launch_bounds__(kBlock) k_batch_assign(BatchMember* __restrict__ mem) {
-    const BatchMember& M = mem[blockIdx.x / kBatchAssignBlocks];
+__global__ void __launch_bounds__(kBlock)
+k_batch_assign(BatchMember* __restrict__ mem, const int32_t* __restrict__ members) {
+    const int slot = blockIdx.x / kBatchAssignBlocks;
+    const BatchMember& M = mem[members ? members[slot] : slot];
     if (!M.assign) return;
     const double* __restrict__ H = M.H[M.hpar];
     double* __restrict__ Ho = M.Hold;
@@ -2039,7 +2061,13 @@ __global__ void __launch_bounds__(kBlock) k_batch_assign(BatchMember* __restrict
 
 hipError_t launch_batch_assign(hipStream_t s, BatchMember* mem, int n) {
     if (n <= 0) return hipSuccess;
-    hipLaunchKernelGGL(k_batch_assign, dim3(n * kBatchAssignBlocks), dim3(kBlock), 0, s, mem);
+    hipLaunchKernelGGL(k_batch_assign, dim3(n * kBatchAssignBlocks), dim3(kBlock), 0, s, mem, nullptr);
+    return hipGetLastError();
+}
+
+hipError_t launch_batch_assign_list(hipStream_t s, BatchMember* mem, const int32_t* members, int n) {
+    if (n <= 0) return hipSuccess;
+    hipLaunchKernelGGL(k_batch_assign, dim3(n * kBatchAssignBlocks), dim3(kBlock), 0, s, mem, members);
     return hipGetLastError();
 }
 
