@@ -1544,8 +1544,9 @@ k_outer_step(const KP<double> p, const double* __restrict__ Hin, double* __restr
         double Hoc = Hin[g];
         double Hnc = Hoc + (double)dl[g];
         double Dc = Dv(g);
-        // Every load of a plane step is unconditional, from an in-range address, and issued before the
-        // arithmetic, so the step waits for its loads once.
+        // The loads of a plane step are unconditional and in range (for a face or cell that does not
+        // exist they read a valid address, the value then unused), so they issue together instead of
+        // each behind its own branch and wait.
         for (int64_t k = k0; k < k1; ++k, g += pl, ox += sx, oy += sy, oz += sz) {
             const bool ki = k >= 1 && k <= nz - 2;
             const bool vx = i >= 1 && ji && ki, vy = j >= 1 && ii && ki, vz = k >= 1 && ii && ji;
