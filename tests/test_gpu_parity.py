@@ -486,3 +486,19 @@ def test_fp32_reference_ic_no_underflow_nan(pkg, orc):
     ref = orc.run_script("diff_3D_nonlin_perf", n, n, n, itmax=100)
     assert np.all(np.isfinite(res.H))
     assert _relmax(res.H, ref["H"]) <= 1e-4
+
+
+@pytest.mark.parametrize("script,shape", [("diff_3D_nonlin_multixpu", (29, 27, 25)), ("diff_2D_linstep", (50, 44)),
+                                          ("diff_1D_nonlin", (64,))])
+def test_itmax_between_checks_matches_oracle(script, shape, pkg, orc):
+    """itMax that is not a multiple of nout: each step runs itMax iterations, the last check at the
+    largest multiple of nout below it sets err, and the loop ends on iter >= itMax (the `while err > tol &&
+    iter < itMax` of scripts/diff_3D/diff_3D_nonlin_multixpu.jl:104), unless a check already met tol (the
+    3D case converges at 20 after its first step); 25 iterations are two-level pairs plus an odd single
+    sweep in the fused paths."""
+    print(f"[gpu] itmax 25 {script} {shape}", flush=True)
+    ref = orc.run_script(script, *shape, itmax=25)
+    res = pkg.run_script(script, *shape, itMax=25, H0=ref["H0"], D=ref["D"])
+    assert list(res.iters) == list(ref["iters"]) and 25 in list(ref["iters"])
+    assert np.array_equal(res.H, ref["H"]), f"rel {_relmax(res.H, ref['H']):.2e}"
+    np.testing.assert_allclose(res.errs, ref["errs"], rtol=1e-12)
