@@ -170,8 +170,8 @@ int member_tiles(const ptd_params& p, int kind, int rows, int tz) {
     return (int)(ntx * nty);
 }
 
-// the single-problem sweeps' z chunk depth (fused_tz): the checking launch must use their tiling
-int check_tz(const ptd_params& p) { return (int)std::min<int64_t>(32, p.nz - 2); }
+// the single-problem sweeps' z chunk depth: the checking launch must use their tiling
+int check_tz(const ptd_params& p) { return ptd::default_chunk_tz(p); }
 
 int upload_table(ptd_batch* b, const std::vector<BatchTile>& t, BatchTile** d) {
     int st = balloc(b, (void**)d, t.size() * sizeof(BatchTile));

@@ -546,6 +546,7 @@ int set_error(int code, const std::string& msg);   // sets ptd_last_error(), ret
 enum { kSweep2D = 2, kSweep3D = 3, kSweep2D2 = 4 };
 bool use_2d_t2(const ptd_params& p);
 int batch_kind(const ptd_params& p);
+int default_chunk_tz(const ptd_params& p);  // z chunk depth of the single-problem 3D sweeps
 // A batched sweep launch over the tiles of every member of one (kind, dcase, arith) group: launch
 // `li` of the check block (buffers selected by the members' parities + li), with rows R per block
 // (two-level 2D: R = rows per tile chunk).

@@ -1113,11 +1113,27 @@ int64_t max_chunk_tz(const ptd_params& p, int64_t esz) {
 // z chunk depth of the fused 3D sweeps: a chunk re-reads and re-computes 2 halo planes, so deeper chunks
 // cost less per plane, while fewer chunks leave fewer blocks to fill 256 CUs. Measured at 511^3 (float64
 // FAST sweep, T_eff GB/s): 8 7296, 16 8054, 24 8159, 28 8259, 32 8450, 36 8223, 40 8242, 48 8207, 64 8317;
-// 32 also gives the fastest float32 correction and EXACT sweeps.
+// 32 also gives the fastest float32 correction and EXACT sweeps. The float64 step-D sweeps hold one
+// 16-row block per CU, and their last round of blocks can leave most CUs idle: they take 24 planes
+// where that needs fewer block-rounds per plane, counting a block's cost as its planes plus 4
+// (measured t_it: 255^3 0.1965 vs 0.2029 ms, 383^3 0.5832 vs 0.5913 ms; 511^3 keeps 32, 1.346 vs
+// 1.386 ms). The batched checking launches tile as this does.
+int default_chunk_tz(const ptd_params& p) {
+    const int64_t mz = p.nz - 2;
+    int tz = (int)std::min<int64_t>(32, mz);
+    if (p.ndim == 3 && p.dcase == PTD_STEP && p.precision == PTD_FP64 && mz > 32) {
+        constexpr int64_t kCUs = 256, kRowsStep = 16;
+        const int64_t tiles = (p.nx - 2 + 59) / 60 * ((p.ny - 2 + kRowsStep - 5) / (kRowsStep - 4));
+        // rounds of blocks times the planes each block marches: the time of the whole sweep
+        auto cost = [&](int64_t d) { return (tiles * ((mz + d - 1) / d) + kCUs - 1) / kCUs * (d + 4); };
+        if (cost(24) < cost(32)) tz = 24;
+    }
+    return tz;
+}
+
 static int fused_tz(const ptd_ctx* c) {
     const char* env = getenv("PTD_TZ");
-    const int64_t mz = c->p.nz - 2;
-    const int64_t want = env && atoi(env) > 0 ? atoi(env) : std::min<int64_t>(32, mz);
+    const int64_t want = env && atoi(env) > 0 ? atoi(env) : default_chunk_tz(c->p);
     return (int)std::max<int64_t>(1, std::min(want, max_chunk_tz(c->p, (int64_t)c->esz)));
 }
 
