@@ -502,3 +502,22 @@ def test_itmax_between_checks_matches_oracle(script, shape, pkg, orc):
     assert list(res.iters) == list(ref["iters"]) and 25 in list(ref["iters"])
     assert np.array_equal(res.H, ref["H"]), f"rel {_relmax(res.H, ref['H']):.2e}"
     np.testing.assert_allclose(res.errs, ref["errs"], rtol=1e-12)
+
+
+def test_mixed_solve_stops_at_itmax(pkg, orc):
+    """itMax bounds the float32 sweeps of a mixed-precision PT solve as it bounds the iterations of the
+    float64 one: the solve returns PTD_OK with its float64 residual above tol, and the next call continues
+    from there."""
+    shape = (45, 38, 29)
+    print(f"[gpu] mixed solve with itMax 20 {shape}", flush=True)
+    spec = pkg.SCRIPTS["diff_3D_nonlin_multixpu"]
+    params = pkg.make_params(spec, *shape, precision=pkg.MIXED)
+    params.itmax = 20
+    H0 = pkg.initial_condition(spec, *shape, kind="config")
+    with pkg.Context(params) as ctx:
+        ctx.set("H", H0)
+        ctx.set("Hold", H0)
+        n1, rms1, _ = ctx.pt_solve()
+        n2, rms2, _ = ctx.pt_solve()
+    assert n1 == 20 and 0 < n2 <= 20
+    assert rms1 > params.tol and rms2 < rms1
