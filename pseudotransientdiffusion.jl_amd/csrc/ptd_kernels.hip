@@ -1501,24 +1501,29 @@ k_outer_residual(const KP<double> p, const double* __restrict__ H, const double*
 // iterate Hn = Hc + delta evaluated on the fly (delta is 0 on the boundary cells), the correction fluxes
 // move from Hc to Hn, Hn goes to the other H buffer, and the residual of the float64 system at Hn gives
 // ResH, the next source R*dt, float32(Hn) and the partials. The next correction's delta = 0 goes to the
-// other delta buffer, so no thread writes what another reads. A thread owns one (i, j) column of a z chunk
-// and marches along z with the z neighbours in registers (each plane of Hc and delta is read once from
-// HBM; the x / y neighbours come from the same plane through the caches).
+// other delta buffer, so no thread writes what another reads. A block owns a 64 x 4 (x, y) tile of a z
+// chunk, a thread one (i, j) column of it, marching along z with the z neighbours in registers. The y
+// neighbours come from the other waves of the tile or, at its edges, from the tiles beside it, which
+// the XCD-aware numbering places on the same XCD at about the same time, so they are L2 hits.
+constexpr int kOuterTx = 64, kOuterTy = kBlock / kOuterTx;
 template <int DC, int FORM, int ARITH>
 __global__ void __launch_bounds__(kBlock)
 k_outer_step(const KP<double> p, const double* __restrict__ Hin, double* __restrict__ Hout,
              const float* __restrict__ dl, float* __restrict__ dl_next, const double* __restrict__ Hold,
              const double* __restrict__ D, double* __restrict__ resh, float* __restrict__ src,
              float* __restrict__ hcf, float* __restrict__ px, float* __restrict__ py, float* __restrict__ pz,
-             double3 sc, double3 isc, double2* __restrict__ partial, int tz) {
+             double3 sc, double3 isc, double2* __restrict__ partial, int tz, int ntx, int ntiles, int per_xcd) {
     using Ph = Phys<double, DC, FORM, ARITH>;
     const int64_t nx = p.nx, ny = p.ny, nz = p.nz, pl = nx * ny;
-    const int64_t nbx = (nx + kBlock - 1) / kBlock;  // blocks per row; blockIdx.x = row block + nbx * j
-    const int64_t i = (int64_t)(blockIdx.x % nbx) * kBlock + threadIdx.x, j = blockIdx.x / nbx;
+    const int b = blockIdx.x;
+    const int tile = (b & 7) * per_xcd + (b >> 3);  // XCD-aware: each XCD walks a contiguous tile range
+    const int tyl = tile / ntx;
+    const int64_t i = (int64_t)(tile - tyl * ntx) * kOuterTx + threadIdx.x;
+    const int64_t j = (int64_t)tyl * kOuterTy + threadIdx.y;
     const int64_t k0 = (int64_t)blockIdx.y * tz, k1 = k0 + tz < nz ? k0 + tz : nz;
     const int64_t part = (int64_t)blockIdx.y * gridDim.x + blockIdx.x;
     double s = 0.0, m = 0.0;
-    if (i < nx) {
+    if (tile < ntiles && i < nx && j < ny) {
         const bool ii = i >= 1 && i <= nx - 2, ji = j >= 1 && j <= ny - 2;
         auto Dv = [&](int64_t o) { return DC == PTD_STEP ? D[o] : 0.0; };
         auto gflux = [&](double Ha, double Hb, double Da, double Db, double ih) {
@@ -1608,25 +1613,29 @@ k_outer_step(const KP<double> p, const double* __restrict__ Hin, double* __restr
             Dc = Dp;
         }
     }
-    block_sum_max(s, m, threadIdx.x, kBlock, &partial[part]);
+    block_sum_max(s, m, threadIdx.y * kOuterTx + threadIdx.x, kBlock, &partial[part]);
 }
 
 template <int DC, int FORM, int ARITH>
 static hipError_t outer_step_t(ptd_ctx* c) {
     const KP<double> kp = make_kp<double>(c->p);
     constexpr int kTzStep = 32;  // planes per thread
-    const dim3 grid((unsigned)((c->p.nx + kBlock - 1) / kBlock * c->p.ny), (unsigned)((c->p.nz + kTzStep - 1) / kTzStep));
+    const int64_t ntx = (c->p.nx + kOuterTx - 1) / kOuterTx, nty = (c->p.ny + kOuterTy - 1) / kOuterTy;
+    const int64_t ntiles = ntx * nty, per = (ntiles + 7) / 8;
+    if (8 * per > (int64_t)INT32_MAX) return hipErrorInvalidValue;
+    const dim3 grid((unsigned)(8 * per), (unsigned)((c->p.nz + kTzStep - 1) / kTzStep));
     const int64_t gb = (int64_t)grid.x * grid.y * grid.z;
     if (gb > c->npartial) return hipErrorInvalidValue;
     const int dc = c->p.dcase;
     const double3 sc = make_double3(corr_flux_scale(c->p, dc, 0), corr_flux_scale(c->p, dc, 1),
                                     corr_flux_scale(c->p, dc, 2));
     const double3 isc = make_double3(1.0 / sc.x, 1.0 / sc.y, 1.0 / sc.z);
-    hipLaunchKernelGGL((k_outer_step<DC, FORM, ARITH>), grid, dim3(kBlock), 0, c->stream, kp,
+    hipLaunchKernelGGL((k_outer_step<DC, FORM, ARITH>), grid, dim3(kOuterTx, kOuterTy), 0, c->stream, kp,
                        (const double*)c->H[c->hcur], (double*)c->H[1 - c->hcur], (const float*)c->dl[c->dcur],
                        (float*)c->dl[1 - c->dcur], (const double*)c->Hold, (const double*)c->D, (double*)c->resh,
                        (float*)c->src, (float*)c->hcf, (float*)c->pq[0][c->pcur], (float*)c->pq[1][c->pcur],
-                       (float*)c->pq[2][c->pcur], sc, isc, (double2*)c->partial, kTzStep);
+                       (float*)c->pq[2][c->pcur], sc, isc, (double2*)c->partial, kTzStep, (int)ntx, (int)ntiles,
+                       (int)per);
     hipLaunchKernelGGL(k_reduce_partials, dim3(1), dim3(kBlock), 0, c->stream, (const double2*)c->partial,
                        (int64_t)gb, c->dred);
     hipError_t e = hipGetLastError();
