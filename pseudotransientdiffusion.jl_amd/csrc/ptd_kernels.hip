@@ -1505,7 +1505,7 @@ k_outer_residual(const KP<double> p, const double* __restrict__ H, const double*
 // chunk, a thread one (i, j) column of it, marching along z with the z neighbours in registers. The y
 // neighbours come from the other waves of the tile or, at its edges, from the tiles beside it, which
 // the XCD-aware numbering places on the same XCD at about the same time, so they are L2 hits.
-constexpr int kOuterTx = 64, kOuterTy = kBlock / kOuterTx;
+constexpr int kOuterTx = 64, kOuterTy = kBlock / kOuterTx, kOuterTz = 32;  // tile and z chunk (planes per thread)
 template <int DC, int FORM, int ARITH>
 __global__ void __launch_bounds__(kBlock)
 k_outer_step(const KP<double> p, const double* __restrict__ Hin, double* __restrict__ Hout,
@@ -1619,11 +1619,10 @@ k_outer_step(const KP<double> p, const double* __restrict__ Hin, double* __restr
 template <int DC, int FORM, int ARITH>
 static hipError_t outer_step_t(ptd_ctx* c) {
     const KP<double> kp = make_kp<double>(c->p);
-    constexpr int kTzStep = 32;  // planes per thread
     const int64_t ntx = (c->p.nx + kOuterTx - 1) / kOuterTx, nty = (c->p.ny + kOuterTy - 1) / kOuterTy;
     const int64_t ntiles = ntx * nty, per = (ntiles + 7) / 8;
     if (8 * per > (int64_t)INT32_MAX) return hipErrorInvalidValue;
-    const dim3 grid((unsigned)(8 * per), (unsigned)((c->p.nz + kTzStep - 1) / kTzStep));
+    const dim3 grid((unsigned)(8 * per), (unsigned)((c->p.nz + kOuterTz - 1) / kOuterTz));
     const int64_t gb = (int64_t)grid.x * grid.y * grid.z;
     if (gb > c->npartial) return hipErrorInvalidValue;
     const int dc = c->p.dcase;
@@ -1634,7 +1633,7 @@ static hipError_t outer_step_t(ptd_ctx* c) {
                        (const double*)c->H[c->hcur], (double*)c->H[1 - c->hcur], (const float*)c->dl[c->dcur],
                        (float*)c->dl[1 - c->dcur], (const double*)c->Hold, (const double*)c->D, (double*)c->resh,
                        (float*)c->src, (float*)c->hcf, (float*)c->pq[0][c->pcur], (float*)c->pq[1][c->pcur],
-                       (float*)c->pq[2][c->pcur], sc, isc, (double2*)c->partial, kTzStep, (int)ntx, (int)ntiles,
+                       (float*)c->pq[2][c->pcur], sc, isc, (double2*)c->partial, kOuterTz, (int)ntx, (int)ntiles,
                        (int)per);
     hipLaunchKernelGGL(k_reduce_partials, dim3(1), dim3(kBlock), 0, c->stream, (const double2*)c->partial,
                        (int64_t)gb, c->dred);
@@ -1733,6 +1732,9 @@ int64_t partial_capacity(const ptd_params& p, int64_t nR) {
         nb = std::max<int64_t>(nb, ntx * nty * (p.nz - 2));
         const int64_t ntx2 = (p.nx - 2 + 59) / 60, nty2 = (p.ny - 2 + kMinRows2 - 5) / (kMinRows2 - 4);
         nb = std::max<int64_t>(nb, ntx2 * nty2 * (p.nz - 2));
+        // k_outer_step: one partial per block, its tiles padded to a multiple of 8 (XCD mapping)
+        const int64_t to = ((p.nx + kOuterTx - 1) / kOuterTx) * ((p.ny + kOuterTy - 1) / kOuterTy);
+        nb = std::max<int64_t>(nb, (to + 7) / 8 * 8 * ((p.nz + kOuterTz - 1) / kOuterTz));
     }
     return std::max<int64_t>(nb, 1);
 }

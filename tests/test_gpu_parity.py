@@ -521,3 +521,27 @@ def test_mixed_solve_stops_at_itmax(pkg, orc):
         n2, rms2, _ = ctx.pt_solve()
     assert n1 == 20 and 0 < n2 <= 20
     assert rms1 > params.tol and rms2 < rms1
+
+
+@pytest.mark.parametrize("shape", [(3, 3, 3), (5, 5, 5), (3, 4, 6), (8, 7, 9), (70, 9, 5), (9, 70, 34)])
+def test_mixed_solve_small_and_ragged_grids(shape, pkg, orc):
+    """The mixed-precision solve on grids down to one inner cell and on thin / ragged boxes (one partial
+    x-y tile of the float64 pass, several z chunks): every step's float64 residual <= tol, and the final H
+    within 1e-7 of the float64 oracle."""
+    print(f"[gpu] mixed small grid {shape}", flush=True)
+    script = "diff_3D_nonlin_multixpu"
+    case = orc.make_case(orc.SCRIPTS[script], *shape)
+    H0 = pkg.initial_condition(pkg.SCRIPTS[script], *shape, kind="config")
+    Hr, Hold_r = H0.copy(order="F"), H0.copy(order="F")
+    _, _, st = orc.solve(case, Hr, Hold_r, None)
+    assert st == 0
+    params = pkg.make_params(pkg.SCRIPTS[script], *shape, precision=pkg.MIXED)
+    with pkg.Context(params) as ctx:
+        ctx.set("H", H0)
+        ctx.set("Hold", H0)
+        for step in range(case.nt):
+            n, rms, mx = ctx.pt_solve()
+            assert rms <= params.tol, (step, rms)
+            ctx.assign()
+        Hm = ctx.get("H")
+    assert _relmax(Hm, Hr) <= 1e-7, _relmax(Hm, Hr)
