@@ -19,6 +19,9 @@ MEMBERS = [
     ("diff_3D_nonlin_multixpu", (67, 35, 21), 0.7, (0.3, 0.7, 0.5)),
     # large enough (17 x 62 tiles) for the two-level 2D sweep
     ("diff_2D_nonlin", (1000, 990), 1.0, (0.5, 0.45)),
+    # the smallest grids: one inner cell / one inner row
+    ("diff_3D_lin_multixpu", (3, 3, 3), 1.0, (0.5, 0.5, 0.5)),
+    ("diff_2D_nonlin", (3, 4), 1.2, (0.5, 0.5)),
 ]
 
 
