@@ -99,3 +99,19 @@ def test_context_ends_as_after_solve(pkg):
             states.append((its, ctx.get("H"), ctx.get("Hold"), ctx.get("qHx"), ctx.get("qHz"), ctx.solve(1)[0]))
     for a, b in zip(*states):
         assert np.array_equal(a, b)
+
+
+@pytest.mark.parametrize("script,shape", [("diff_3D_linstep_multixpu", (3, 3, 3)), ("diff_3D_lin_multixpu", (5, 6, 3)),
+                                          ("diff_2D_linstep", (4, 3))])
+def test_gradient_on_the_smallest_grids(script, shape, pkg):
+    """One inner cell / one inner row: the adjoint source, the face gather into the cells and the boundary
+    coupling of dL/dH0 on grids whose every face touches the boundary."""
+    print(f"[gpu] gradient on {script} {shape}", flush=True)
+    nt = 3
+    p, H0, D, Hobs, h = _setup(pkg, script, shape, tol=1e-12)
+    g = _grad_gpu(pkg, p, H0, D, Hobs, nt)
+    be = BELinear(D, h, p.dt)
+    L, gD, gH0 = be.gradient(H0, Hobs, nt)
+    assert abs(g["loss"] - L) <= 1e-9 * max(L, 1e-300)
+    assert np.max(np.abs(g["grad_D"] - gD)) <= 1e-6 * max(np.max(np.abs(gD)), 1e-300)
+    assert np.max(np.abs(g["grad_H0"] - gH0)) <= 1e-6 * max(np.max(np.abs(gH0)), 1e-300)
