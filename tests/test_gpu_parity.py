@@ -545,3 +545,34 @@ def test_mixed_solve_small_and_ragged_grids(shape, pkg, orc):
             ctx.assign()
         Hm = ctx.get("H")
     assert _relmax(Hm, Hr) <= 1e-7, _relmax(Hm, Hr)
+
+
+@pytest.mark.parametrize("script", FUSED_SCRIPTS)
+def test_two_level_default_chunks_match_oracle(script, pkg, orc, monkeypatch):
+    """EXACT arithmetic with the library's own z chunk depth on a tall grid: 98 inner planes are three
+    32-plane chunks and a 2-plane one (nonlinear / linear), or four 24-plane chunks and a 2-plane one (step
+    D), and 6 iterations give the oracle's values bit for bit."""
+    shape = (20, 18, 100)
+    print(f"[gpu] default chunks {script} {shape}", flush=True)
+    monkeypatch.delenv("PTD_TZ", raising=False)
+    params, case, H, Hold, D, q = _random_state(pkg, orc, script, shape, seed=7)
+    with pkg.Context(params) as ctx:
+        ctx.set("H", H)
+        ctx.set("Hold", Hold)
+        for k, v in q.items():
+            ctx.set(k, v)
+        dtau = None
+        if D is not None:
+            ctx.set("D", D)
+            dtau = np.zeros(pkg.field_shape(params, 10), order="F")
+            orc.lib().ptdo_iter_params(orc.ctypes.byref(case), orc._p(D), orc._p(dtau))
+        ctx.iterate(6)
+        Hg = ctx.get("H")
+        qg = {k: ctx.get(k) for k in q}
+    Hr = H.copy(order="F")
+    qr = {k: v.copy(order="F") for k, v in q.items()}
+    orc.lib().ptdo_iterate(orc.ctypes.byref(case), orc._p(Hr), orc._p(Hold), orc._p(D), orc._p(dtau),
+                           orc._p(qr["qHx"]), orc._p(qr["qHy"]), orc._p(qr["qHz"]), 6)
+    assert np.array_equal(Hg, Hr), f"H: rel {_relmax(Hg, Hr):.2e}"
+    for k in q:
+        assert np.array_equal(qg[k], qr[k]), f"{k}: rel {_relmax(qg[k], qr[k]):.2e}"
