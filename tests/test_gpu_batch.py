@@ -108,3 +108,35 @@ def test_batch_workload_sample(pkg):
     assert np.all(it > 0) and np.all(it % 10 == 0)
     assert np.all(er <= 1e-8)
     assert 0 < B.percentile(lat, 50) <= B.percentile(lat, 99) <= total * 1.01 + 1.0
+
+
+def test_batch_solves_with_growing_step_counts(pkg):
+    """A second solve call with more physical steps than the first reallocates the per-step outputs and
+    re-captures the check-block graphs; every member still equals its single solves over the same calls."""
+    print("[gpu] batch solve(2) then solve(4)", flush=True)
+    members = [("diff_3D_nonlin_multixpu", (28, 24, 22), 1.0, (0.5, 0.5, 0.5)),
+               ("diff_2D_linstep", (70, 60), 1.5, (0.4, 0.6))]
+    params, fields = [], []
+    for script, shape, amp, centre in members:
+        spec = pkg.SCRIPTS[script]
+        params.append(pkg.make_params(spec, *shape, arith=1))
+        fields.append((_gauss(spec, shape, amp, centre),
+                       pkg.step_field(spec, *shape) if spec["dcase"] == pkg.STEP else None))
+    with pkg.Batch(params) as b:
+        for m, (H0, D) in enumerate(fields):
+            b.set(m, "H", H0)
+            b.set(m, "Hold", H0)
+            if D is not None:
+                b.set(m, "D", D)
+        runs = [b.solve(nt)[:2] for nt in (2, 4)]
+        Hb = [b.get(m, "H") for m in range(len(params))]
+    for m, (H0, D) in enumerate(fields):
+        with pkg.Context(params[m]) as ctx:
+            ctx.set("H", H0)
+            ctx.set("Hold", H0)
+            if D is not None:
+                ctx.set("D", D)
+            for call, nt in enumerate((2, 4)):
+                it, er = ctx.solve(nt)
+                assert np.array_equal(runs[call][0][m], it) and np.array_equal(runs[call][1][m], er)
+            assert np.array_equal(Hb[m], ctx.get("H"))
