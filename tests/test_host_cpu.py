@@ -137,3 +137,20 @@ def test_save_viz_skips_reaction_scripts(tmp_path):
     r = scripts.ScriptResult(name="diff_1D_react", coords=(np.zeros(4),), H=np.zeros(4),
                              iters=np.zeros(1, np.int64), errs=np.zeros(1))
     assert scripts.save_viz("diff_1D_react", r, out_dir=str(tmp_path)) is None
+
+
+def test_batch_workload_draw(pkg):
+    """The batched workload: 96 members from PCG64(0), reproducible, with every field inside the ranges
+    the workload states (2D {128, ..., 1024}^2, 3D {64, ..., 256}^3, D from lin / step / H^3, amplitude
+    U(0.5, 2), centre U(0.3, 0.7) per axis)."""
+    B = pkg.batch
+    w = B.workload(96, 0)
+    assert [(m.script, m.n, m.amp, m.centre) for m in w] == [(m.script, m.n, m.amp, m.centre) for m in B.workload(96, 0)]
+    assert len(w) == 96
+    kinds = {(pkg.SCRIPTS[m.script]["ndim"], pkg.SCRIPTS[m.script]["dcase"]) for m in w}
+    assert kinds == {(d, c) for d in (2, 3) for c in (pkg.LIN, pkg.STEP, pkg.NONLIN)}
+    for m in w:
+        nd = pkg.SCRIPTS[m.script]["ndim"]
+        assert m.n in (B.SIZES_2D if nd == 2 else B.SIZES_3D)
+        assert 0.5 <= m.amp < 2.0 and len(m.centre) == nd and all(0.3 <= c < 0.7 for c in m.centre)
+    assert B.SIZES_2D == list(range(128, 1025, 64)) and B.SIZES_3D == list(range(64, 257, 32))
