@@ -132,6 +132,16 @@ int balloc(ptd_batch* b, void** ptr, size_t bytes) {
     return PTD_OK;
 }
 
+// frees one of the batch's allocations (null: nothing)
+void release(ptd_batch* b, void* ptr, size_t bytes) {
+    if (!ptr) return;
+    auto it = std::find(b->allocs.begin(), b->allocs.end(), ptr);
+    if (it == b->allocs.end()) return;
+    b->allocs.erase(it);
+    b->bytes -= (int64_t)bytes;
+    (void)hipFree(ptr);
+}
+
 void destroy_graphs(ptd_batch* b) {
     for (int c = 0; c < kNumCandidates; ++c) {
         if (b->exec[c]) (void)hipGraphExecDestroy(b->exec[c]);
@@ -464,10 +474,13 @@ int32_t ptd_batch_solve(ptd_batch* b, int32_t nt, int64_t* iters, double* errs, 
         int st = balloc(b, (void**)&di, (size_t)b->n * nt * sizeof(int64_t));
         if (st == PTD_OK) st = balloc(b, (void**)&de, (size_t)b->n * nt * sizeof(double));
         if (st != PTD_OK) return st;
+        PTD_HIPB(hipStreamSynchronize(b->stream));  // no replay may still use the old buffers
+        destroy_graphs(b);                            // they hold the old addresses
+        release(b, b->d_iters, (size_t)b->n * b->nt_cap * sizeof(int64_t));
+        release(b, b->d_errs, (size_t)b->n * b->nt_cap * sizeof(double));
         b->d_iters = di;
         b->d_errs = de;
         b->nt_cap = nt;
-        destroy_graphs(b);
     }
     for (auto& M : b->hm) {
         M.active = 1;
