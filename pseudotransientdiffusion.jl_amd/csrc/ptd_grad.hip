@@ -14,6 +14,7 @@
 #include <hip/hip_runtime.h>
 
 #include <cmath>
+#include <cstdint>
 #include <string>
 #include <vector>
 
@@ -55,8 +56,10 @@ extern "C" int32_t ptd_grad_solve(ptd_ctx* c, int32_t nt, const double* h_obs, i
     if (c->p.precision != PTD_FP64) return ptd::set_error(PTD_ERR_STATE, "gradients are float64");
     if (hipSetDevice(c->device) != hipSuccess) return ptd::set_error(PTD_ERR_HIP, "hipSetDevice failed");
 
-    int status = PTD_OK;
     const size_t nH = (size_t)c->nH;
+    if ((size_t)nt + 1 > SIZE_MAX / (nH * 8))
+        return ptd::set_error(PTD_ERR_NOMEM, "nt + 1 checkpoints of the grid exceed the address space");
+    int status = PTD_OK;
     double *ckpt = nullptr, *hobs = nullptr, *gD = nullptr, *gH0 = nullptr;
     double* qbak[3] = {nullptr, nullptr, nullptr};
     double red[2] = {0, 0};
