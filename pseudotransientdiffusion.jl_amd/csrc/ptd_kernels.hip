@@ -1113,21 +1113,34 @@ int64_t max_chunk_tz(const ptd_params& p, int64_t esz) {
 // z chunk depth of the fused 3D sweeps: a chunk re-reads and re-computes 2 halo planes, so deeper chunks
 // cost less per plane, while fewer chunks leave fewer blocks to fill 256 CUs. Measured at 511^3 (float64
 // FAST sweep, T_eff GB/s): 8 7296, 16 8054, 24 8159, 28 8259, 32 8450, 36 8223, 40 8242, 48 8207, 64 8317;
-// 32 also gives the fastest float32 correction and EXACT sweeps. The float64 step-D sweeps hold one
-// 16-row block per CU, and their last round of blocks can leave most CUs idle: they take 24 planes
-// where that needs fewer block-rounds per plane, counting a block's cost as its planes plus 4
-// (measured t_it: 255^3 0.1965 vs 0.2029 ms, 383^3 0.5832 vs 0.5913 ms; 511^3 keeps 32, 1.346 vs
-// 1.386 ms). The batched checking launches tile as this does.
+// 32 also gives the fastest float32 correction and EXACT sweeps. The time of a sweep is modelled as the
+// rounds of blocks the GPU runs times the planes a block marches (its chunk plus 4): 256 blocks per round
+// for the float64 step-D sweeps (one 16-row block per CU), 512 for the others (two per CU).
+//  - Grids whose 32-plane tiling fills fewer than two rounds take the depth of least modelled time among
+//    32, 24, 16, 12, 8, 4 (measured 5-step solves, ms: 64^3 nonlinear 7.0 -> 1.9 at 4 planes; 128^3
+//    nonlinear 12.6 -> 11.0 at 16, linear 25.7 -> 18.9 at 12, step 35.4 -> 28.1 at 24; 192^3 nonlinear
+//    46.5 -> 40.1 at 12).
+//  - The step-D sweeps also choose between 24 and 32 on larger grids, where their last round of blocks
+//    can leave most CUs idle (t_it: 255^3 0.1965 vs 0.2029 ms, 383^3 0.5832 vs 0.5913 ms; 511^3 keeps 32,
+//    1.346 vs 1.386 ms).
+// The batched checking launches tile as this does.
 int default_chunk_tz(const ptd_params& p) {
     const int64_t mz = p.nz - 2;
     int tz = (int)std::min<int64_t>(32, mz);
-    if (p.ndim == 3 && p.dcase == PTD_STEP && p.precision == PTD_FP64 && mz > 32) {
-        constexpr int64_t kCUs = 256, kRowsStep = 16;
-        const int64_t tiles = (p.nx - 2 + 59) / 60 * ((p.ny - 2 + kRowsStep - 5) / (kRowsStep - 4));
-        // rounds of blocks times the planes each block marches: the time of the whole sweep
-        auto cost = [&](int64_t d) { return (tiles * ((mz + d - 1) / d) + kCUs - 1) / kCUs * (d + 4); };
-        if (cost(24) < cost(32)) tz = 24;
+    if (p.ndim != 3) return tz;
+    const bool step64 = p.dcase == PTD_STEP && p.precision == PTD_FP64;
+    const int64_t slots = step64 ? 256 : 512;
+    const int64_t rows = step64 || p.dcase == PTD_LIN || p.precision != PTD_FP64 ? 12 : 8;  // new rows per tile
+    const int64_t tiles = (p.nx - 2 + 59) / 60 * ((p.ny - 2 + rows - 1) / rows);
+    auto blocks = [&](int64_t d) { return tiles * ((mz + d - 1) / d); };
+    auto cost = [&](int64_t d) { return (blocks(d) + slots - 1) / slots * (d + 4); };
+    if (blocks(tz) < 2 * slots) {
+        int64_t best = tz;
+        for (int64_t d : {24, 16, 12, 8, 4})
+            if (d < tz && cost(d) < cost(best)) best = d;
+        return (int)best;
     }
+    if (step64 && mz > 32 && cost(24) < cost(32)) tz = 24;
     return tz;
 }
 

@@ -547,12 +547,12 @@ def test_mixed_solve_small_and_ragged_grids(shape, pkg, orc):
     assert _relmax(Hm, Hr) <= 1e-7, _relmax(Hm, Hr)
 
 
+@pytest.mark.parametrize("shape", [(20, 18, 100), (481, 258, 100)])
 @pytest.mark.parametrize("script", FUSED_SCRIPTS)
-def test_two_level_default_chunks_match_oracle(script, pkg, orc, monkeypatch):
-    """EXACT arithmetic with the library's own z chunk depth on a tall grid: 98 inner planes are three
-    32-plane chunks and a 2-plane one (nonlinear / linear), or four 24-plane chunks and a 2-plane one (step
-    D), and 6 iterations give the oracle's values bit for bit."""
-    shape = (20, 18, 100)
+def test_two_level_default_chunks_match_oracle(script, shape, pkg, orc, monkeypatch):
+    """EXACT arithmetic with the library's own z chunk depth on tall grids: 98 inner planes are 4-plane
+    chunks and a 2-plane one on the narrow grid (too few tiles to fill the GPU otherwise), and 32- or 24-plane
+    chunks and a 2-plane one on the wide grid; 6 iterations give the oracle's values bit for bit."""
     print(f"[gpu] default chunks {script} {shape}", flush=True)
     monkeypatch.delenv("PTD_TZ", raising=False)
     params, case, H, Hold, D, q = _random_state(pkg, orc, script, shape, seed=7)
