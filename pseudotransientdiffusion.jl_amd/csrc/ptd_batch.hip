@@ -181,7 +181,7 @@ int member_tiles(const ptd_params& p, int kind, int rows, int tz) {
 }
 
 // the single-problem sweeps' z chunk depth: the checking launch must use their tiling
-int check_tz(const ptd_params& p) { return ptd::default_chunk_tz(p); }
+int check_tz(const ptd_params& p) { return ptd::check_chunk_tz(p); }
 
 int upload_table(ptd_batch* b, const std::vector<BatchTile>& t, BatchTile** d) {
     int st = balloc(b, (void**)d, t.size() * sizeof(BatchTile));

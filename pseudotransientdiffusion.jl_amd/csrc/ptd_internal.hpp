@@ -547,6 +547,7 @@ enum { kSweep2D = 2, kSweep3D = 3, kSweep2D2 = 4 };
 bool use_2d_t2(const ptd_params& p);
 int batch_kind(const ptd_params& p);
 int default_chunk_tz(const ptd_params& p);  // z chunk depth of the single-problem 3D sweeps
+int check_chunk_tz(const ptd_params& p);    // ... of their checking launches (the batched ones follow)
 // A batched sweep launch over the tiles of every member of one (kind, dcase, arith) group: launch
 // `li` of the check block (buffers selected by the members' parities + li), with rows R per block
 // (two-level 2D: R = rows per tile chunk).

@@ -1117,36 +1117,53 @@ int64_t max_chunk_tz(const ptd_params& p, int64_t esz) {
 // rounds of blocks the GPU runs times the planes a block marches (its chunk plus 4): 256 blocks per round
 // for the float64 step-D sweeps (one 16-row block per CU), 512 for the others (two per CU).
 //  - Grids whose 32-plane tiling fills fewer than two rounds take the depth of least modelled time among
-//    32, 24, 16, 12, 8, 4 (measured 5-step solves, ms: 64^3 nonlinear 7.0 -> 1.9 at 4 planes; 128^3
-//    nonlinear 12.6 -> 11.0 at 16, linear 25.7 -> 18.9 at 12, step 35.4 -> 28.1 at 24; 192^3 nonlinear
-//    46.5 -> 40.1 at 12).
+//    32, 24, 16, 12, 8, 4 for their non-checking launches (measured 5-step solves, ms: 64^3 nonlinear
+//    7.0 -> 3.0 at 4 planes; 128^3 nonlinear 12.7 -> 11.8 at 16, linear 25.6 -> 19.3 at 12, step
+//    35.6 -> 28.1 at 24; 192^3 nonlinear 46.3 -> 41.1 at 12).
 //  - The step-D sweeps also choose between 24 and 32 on larger grids, where their last round of blocks
 //    can leave most CUs idle (t_it: 255^3 0.1965 vs 0.2029 ms, 383^3 0.5832 vs 0.5913 ms; 511^3 keeps 32,
 //    1.346 vs 1.386 ms).
-// The batched checking launches tile as this does.
-int default_chunk_tz(const ptd_params& p) {
-    const int64_t mz = p.nz - 2;
-    int tz = (int)std::min<int64_t>(32, mz);
-    if (p.ndim != 3) return tz;
-    const bool step64 = p.dcase == PTD_STEP && p.precision == PTD_FP64;
-    const int64_t slots = step64 ? 256 : 512;
-    const int64_t rows = step64 || p.dcase == PTD_LIN || p.precision != PTD_FP64 ? 12 : 8;  // new rows per tile
-    const int64_t tiles = (p.nx - 2 + 59) / 60 * ((p.ny - 2 + rows - 1) / rows);
-    auto blocks = [&](int64_t d) { return tiles * ((mz + d - 1) / d); };
-    auto cost = [&](int64_t d) { return (blocks(d) + slots - 1) / slots * (d + 4); };
-    if (blocks(tz) < 2 * slots) {
-        int64_t best = tz;
-        for (int64_t d : {24, 16, 12, 8, 4})
-            if (d < tz && cost(d) < cost(best)) best = d;
-        return (int)best;
+// The checking launches keep 32 planes (24 for step D where the model prefers it) on small grids too:
+// their tiling fixes the order in which the residual partials are summed, which batched solves, where
+// the many members fill the GPU anyway, reproduce bit for bit (check_chunk_tz).
+namespace {
+struct ChunkModel {
+    int64_t mz, slots, tiles;
+    bool step64;
+    explicit ChunkModel(const ptd_params& p)
+        : mz(p.nz - 2), slots(p.dcase == PTD_STEP && p.precision == PTD_FP64 ? 256 : 512), tiles(0),
+          step64(p.dcase == PTD_STEP && p.precision == PTD_FP64) {
+        const int64_t rows = step64 || p.dcase == PTD_LIN || p.precision != PTD_FP64 ? 12 : 8;  // new rows per tile
+        tiles = (p.nx - 2 + 59) / 60 * ((p.ny - 2 + rows - 1) / rows);
     }
-    if (step64 && mz > 32 && cost(24) < cost(32)) tz = 24;
-    return tz;
+    int64_t blocks(int64_t d) const { return tiles * ((mz + d - 1) / d); }
+    int64_t cost(int64_t d) const { return (blocks(d) + slots - 1) / slots * (d + 4); }
+};
+}  // namespace
+
+int check_chunk_tz(const ptd_params& p) {
+    const int64_t mz = p.nz - 2;
+    const int tz = (int)std::min<int64_t>(32, mz);
+    if (p.ndim != 3 || mz <= 32) return tz;
+    const ChunkModel m(p);
+    return m.step64 && m.cost(24) < m.cost(32) ? 24 : tz;
 }
 
-static int fused_tz(const ptd_ctx* c) {
+int default_chunk_tz(const ptd_params& p) {
+    const int tz = check_chunk_tz(p);
+    if (p.ndim != 3) return tz;
+    const ChunkModel m(p);
+    const int64_t t32 = std::min<int64_t>(32, m.mz);
+    if (m.blocks(t32) >= 2 * m.slots) return tz;
+    int64_t best = t32;
+    for (int64_t d : {24, 16, 12, 8, 4})
+        if (d < t32 && m.cost(d) < m.cost(best)) best = d;
+    return (int)best;
+}
+
+static int fused_tz(const ptd_ctx* c, bool check) {
     const char* env = getenv("PTD_TZ");
-    const int64_t want = env && atoi(env) > 0 ? atoi(env) : default_chunk_tz(c->p);
+    const int64_t want = env && atoi(env) > 0 ? atoi(env) : (check ? check_chunk_tz(c->p) : default_chunk_tz(c->p));
     return (int)std::max<int64_t>(1, std::min(want, max_chunk_tz(c->p, (int64_t)c->esz)));
 }
 
@@ -1166,7 +1183,7 @@ template <typename T, int DC, int FORM, int ARITH>
 static hipError_t sweep_run(ptd_ctx* c, const SweepBufs& b, bool two, bool check) {
     using Ta = typename AuxOf<T, DC>::type;
     const KP<T> kp = make_kp<T>(c->p);
-    const int tz = fused_tz(c);
+    const int tz = fused_tz(c, check);
     constexpr int R2 = t2_rows<T, DC, ARITH, false>(), R2C = t2_rows<T, DC, ARITH, true>();
     const int brows = two ? (check ? R2C : R2) : kRows;
     const int w = two ? 60 : 62, rows = two ? brows - 4 : brows - 2;
