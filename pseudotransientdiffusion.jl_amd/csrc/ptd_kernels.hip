@@ -1392,8 +1392,13 @@ bool use_2d_t2(const ptd_params& p) {
     return ntx * nty >= 1024;
 }
 int batch_kind(const ptd_params& p) { return p.ndim == 3 ? kSweep3D : (use_2d_t2(p) ? kSweep2D2 : kSweep2D); }
+// PTD_T2=2 in the environment also runs the single-problem 2D solves on two-level sweeps below that
+// threshold (the same bits as single sweeps; lets the tests compare them with the oracle on small grids).
+// Batches keep the threshold.
 bool fused2_available(const ptd_ctx* c) {
-    return fused_available(c) && (c->p.ndim == 3 || use_2d_t2(c->p));
+    const char* env = getenv("PTD_T2");
+    const bool force2d = env && env[0] == '2';
+    return fused_available(c) && (c->p.ndim == 3 || use_2d_t2(c->p) || (c->p.ndim == 2 && force2d));
 }
 
 // ---------------------------------------------------------------------------------------------
