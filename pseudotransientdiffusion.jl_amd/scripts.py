@@ -312,9 +312,86 @@ def save_viz(name, result, out_dir="out_visu"):
     return path
 
 
+def text_file_name(name):
+    """File the script's DO_SAVE row is appended to: the multi-xPU 3D converging scripts drop the
+    `_multixpu` suffix (scripts/diff_3D/diff_3D_lin_multixpu.jl:119, diff_3D_linstep_multixpu.jl:151,
+    diff_3D_nonlin_multixpu.jl:131), the perf scripts keep their full name
+    (scripts/diff_3D/diff_3D_nonlin_multixpu_perf.jl:145, diff_3D_nonlin_perf.jl:127), the 1D / 2D scripts
+    use theirs (scripts/diff_1D_lin.jl:86, scripts/diff_2D_lin.jl:88)."""
+    if name.endswith("_multixpu"):
+        name = name[: -len("_multixpu")]
+    return f"out_{name}.txt"
+
+
+def julia_float(x):
+    """A Float64 as Julia's string interpolation writes it: the shortest round-trip digits, `1.0e-5`-style
+    exponents below 1e-4."""
+    r = repr(float(x))
+    if "e" not in r:
+        return r
+    mant, exp = r.split("e")
+    if "." not in mant:
+        mant += ".0"
+    sign = "-" if exp.startswith("-") else ""
+    return f"{mant}e{sign}{exp.lstrip('+-').lstrip('0') or '0'}"
+
+
+def output_row(name, result, n):
+    """The line the script appends with DO_SAVE:
+    converging 1D / 2D / 3D: `nx [ny [nz]] ittot nt`  (scripts/diff_1D_lin.jl:87, diff_2D_lin.jl:89,
+                                                       diff_3D/diff_3D_nonlin_multixpu.jl:132)
+    2D perf:    `nx ny ittot t_toc A_eff t_it T_eff`    (scripts/diff_2D_nonlin_perf.jl:117)
+    3D perf:    `1 nx ny nz ittot t_toc A_eff t_it T_eff` (scripts/diff_3D/diff_3D_nonlin_perf.jl:128)
+    3D multi-xPU perf: `nprocs nx_g ny_g nz_g ittot ...` with nprocs = 1 here
+                                                      (scripts/diff_3D/diff_3D_nonlin_multixpu_perf.jl:146)."""
+    spec = SCRIPTS[name]
+    dims = " ".join(str(int(v)) for v in n)
+    if spec.get("perf"):
+        prefix = "1 " if spec["ndim"] == 3 else ""
+        vals = " ".join(julia_float(v) for v in (result.t_toc, result.A_eff, result.t_it, result.T_eff))
+        return f"{prefix}{dims} {result.ittot} {vals}"
+    return f"{dims} {result.ittot} {len(result.iters)}"
+
+
+def analytic_error_1d(result, spec):
+    """norm(H - Hana)/sqrt(nx) with Hana = 1/sqrt(4(ttot + 1/4)) exp(-(xc - lx/2)^2/(4 D (ttot + 1/4))), the
+    linear 1D script's error against the analytic solution (scripts/diff_1D_lin.jl:80-81)."""
+    xc = result.coords[0]
+    ttot, lx, D = spec["ttot"], spec["lx"], spec["D"]
+    hana = 1 / math.sqrt(4 * (ttot + 1 / 4)) * np.exp(-(xc - lx / 2) ** 2 / (4 * D * (ttot + 1 / 4)))
+    return float(np.linalg.norm(result.H - hana) / math.sqrt(len(xc)))
+
+
+def _round_sig(x, sig):
+    return float(f"{x:.{sig}g}")
+
+
+def summary_lines(name, result, n):
+    """The lines the script prints after its loop (scripts/diff_3D/diff_3D_nonlin_perf.jl:117-118,
+    scripts/diff_1D_lin.jl:81, scripts/diff_1D_react.jl:70, scripts/diff_2D_react.jl:74)."""
+    spec = SCRIPTS[name]
+    out = []
+    if result.T_eff is not None:
+        out.append("PERF: Time = %1.3f sec, T_eff = %1.2f GB/s (niter = %d)"
+                   % (result.t_toc, _round_sig(result.T_eff, 3), result.niter))
+    if spec.get("react"):
+        if spec["ndim"] == 1:
+            out.append("nx = %d, iterations tot = %d " % (n[0], result.ittot))
+        else:
+            out.append("nx = %d, ny = %d, iterations tot = %d " % (n[0], n[1], result.ittot))
+        return out
+    line = "Total time = %1.2f, time steps = %d, nx = %d, iterations tot = %d" % (
+        _round_sig(spec["ttot"], 2), len(result.iters), n[0], result.ittot)
+    if name == "diff_1D_lin":
+        line += ", error vs analytic = %1.2e" % analytic_error_1d(result, spec)
+    out.append(line + " ")
+    return out
+
+
 def main(argv=None):
-    """CLI mirror of running a script: `python -m <pkg>.scripts diff_3D_nonlin_perf` with NX/NY/NZ and
-    DO_SAVE from the environment (scripts/diff_3D/diff_3D_nonlin_perf.jl:1-8)."""
+    """CLI mirror of running a script: `python -m <pkg>.scripts diff_3D_nonlin_perf` with NX/NY/NZ, DO_SAVE
+    and DO_SAVE_VIZ from the environment (scripts/diff_3D/diff_3D_nonlin_perf.jl:1-8): prints the script's
+    summary lines, appends its row to output/<text_file_name> and writes out_visu/<viz_file_name>."""
     argv = sys.argv[1:] if argv is None else argv
     if not argv or argv[0] not in SCRIPTS:
         print("usage: scripts.py <" + "|".join(SCRIPTS) + ">", file=sys.stderr)
@@ -326,23 +403,14 @@ def main(argv=None):
     nx = int(os.environ.get("NX", default_n))
     ny = int(os.environ.get("NY", default_n))
     nz = int(os.environ.get("NZ", default_n))
+    n = [nx, ny, nz][:nd]
     r = run_script(name, nx, ny, nz)
-    if r.T_eff is not None:
-        print("PERF: Time = %1.3f sec, T_eff = %1.2f GB/s (niter = %d)" % (r.t_toc, float("%.3g" % r.T_eff), r.niter))
-    if spec.get("react"):
-        print("nx = %d, iterations tot = %d " % (nx, r.ittot))
-    else:
-        ttot = spec["ttot"]
-        print("Total time = %1.2f, time steps = %d, nx = %d, iterations tot = %d " % (ttot, len(r.iters), nx, r.ittot))
+    for line in summary_lines(name, r, n):
+        print(line)
     if _env_bool("DO_SAVE", False) and not spec.get("react"):
         os.makedirs("output", exist_ok=True)
-        n = [nx, ny, nz][:nd]
-        with open(os.path.join("output", f"out_{name}.txt"), "a") as f:
-            if r.T_eff is not None:   # perf rows (scripts/diff_3D/diff_3D_nonlin_perf.jl:128)
-                prefix = "1 " if nd == 3 else ""
-                f.write(prefix + " ".join(str(v) for v in n) + f" {r.ittot} {r.t_toc} {r.A_eff} {r.t_it} {r.T_eff}\n")
-            else:                     # converging rows (scripts/diff_3D/diff_3D_nonlin_multixpu.jl:132)
-                f.write(" ".join(str(v) for v in n) + f" {r.ittot} {len(r.iters)}\n")
+        with open(os.path.join("output", text_file_name(name)), "a") as f:
+            f.write(output_row(name, r, n) + "\n")
     if _env_bool("DO_SAVE_VIZ", False):
         save_viz(name, r)
     return 0

@@ -76,3 +76,16 @@ def sample(H, n, ndim):
 def isapprox_norm(a, b):
     """Julia isapprox(a, b) with default rtol = sqrt(eps): norm(a-b) <= rtol*max(norm(a), norm(b))."""
     return np.linalg.norm(a - b) <= np.sqrt(np.finfo(float).eps) * max(np.linalg.norm(a), np.linalg.norm(b))
+
+
+# The oracle's per-step PT iteration counts of the 1D scripts at n = 16384 with its own initial condition
+# (glibc exp) and row-ordered residual sums; recomputed on the CPU by
+# tests/test_oracle_golden.py::test_oracle_1d_16384_counts. At this size the residual sits on a roundoff
+# plateau within a few percent of tol for hundreds of iterations, so the stopping check depends on the
+# last-ulp history: the reference's published totals (63530, 64460, 30430; output/out_diff_1D_*.txt:10)
+# come from other arithmetic (Julia's exp, CUDA, CUBLAS nrm2) and are not reproducible at the count level.
+ORACLE_1D_16384 = {
+    "1D_lin": [13790, 12910, 12590, 12300, 12080],
+    "1D_linstep": [14970, 12710, 12340, 12270, 12190],
+    "1D_nonlin": [7410, 6290, 5830, 5590, 5340],
+}

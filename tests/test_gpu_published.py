@@ -1,8 +1,17 @@
 """The GPU path against the reference's published iteration counts (output/out_diff_{1,2,3}D_*.txt, runs of
-the reference on GPUs, nout = 10) at the published grid sizes: 1D up to 16384, 2D up to 2048^2, 3D up to
+the reference on GPUs, nout = 10) at the published grid sizes: 1D up to 8192, 2D up to 2048^2, 3D up to
 510^3. The loop-level entry points in EXACT arithmetic must need exactly the published number of PT
-iterations over the 5 physical steps (1D at n = 16384: within 0.2%, see the test)."""
+iterations over the 5 physical steps.
+
+1D at n = 16384: the residual sits on a roundoff plateau within a few percent of tol for hundreds of
+iterations before it crosses, so the stopping check depends on the last-ulp history of the run. One-ulp
+differences in H0 alone (numpy's exp against glibc's, 760 of 16384 cells) move the totals by up to 50
+iterations, and FMA contraction by up to 120; the reference's totals come from Julia's exp, CUDA
+arithmetic and CUBLAS nrm2. There the test feeds the oracle's own H0 and requires the oracle's per-step
+counts exactly (pinned on the CPU by test_oracle_golden.py::test_oracle_1d_16384_counts)."""
 import pytest
+
+from conftest import ORACLE_1D_16384
 
 pytestmark = pytest.mark.gpu
 
@@ -25,7 +34,7 @@ def _cases():
 
 
 @pytest.mark.parametrize("key,n,ittot,nt", _cases())
-def test_gpu_published_iteration_counts(key, n, ittot, nt, pkg):
+def test_gpu_published_iteration_counts(key, n, ittot, nt, pkg, orc):
     nd = int(key[0])
     case = key.split("_", 1)[1]
     script = f"diff_{nd}D_{case}" + ("_multixpu" if nd == 3 else "")
@@ -36,15 +45,15 @@ def test_gpu_published_iteration_counts(key, n, ittot, nt, pkg):
         nloc = (n - 2) // 2 + 2
         over["norm_len"] = float(((nloc - 4) * 2 + 2) ** 3)
     print(f"[gpu] published {key} n={n}", flush=True)
+    if nd == 1 and n >= 16384:
+        spec = dict(orc.SCRIPTS[script])
+        case = orc.make_case(spec, n)
+        H0 = orc.initial_condition(spec["ic"], case)
+        D = orc.step_field(spec["dstep"], case, spec["d1"], spec["d2"]) if spec["dcase"] == orc.STEP else None
+        r = pkg.run_script(script, n, H0=H0, D=D)
+        assert [int(v) for v in r.iters] == ORACLE_1D_16384[key], f"{key} n={n}: {list(r.iters)}"
+        return
     r = pkg.run_script(script, n, n, n, **over)
     assert len(r.iters) == nt
     assert all(int(v) % 10 == 0 for v in r.iters)
-    if nd == 1 and n >= 16384:
-        # The 1D tables come from the reference's CUDA runs (USE_GPU=true, scripts/runme_diff1D.sh:14),
-        # while this path follows the reference's CPU arithmetic bit for bit (the reftest fixtures);
-        # a likely difference is CUDA's default contraction of a*b+c into FMA. Over the 60000+ PT
-        # iterations at n = 16384 the counts drift by at most 0.16% (measured: +100, +10, +40); every
-        # smaller 1D size and every 2D / 3D size here matches exactly.
-        assert abs(r.ittot - ittot) <= 0.002 * ittot, f"{key} n={n}: {r.ittot} vs published {ittot}"
-    else:
-        assert r.ittot == ittot, f"{key} n={n}: {r.ittot} iterations, published {ittot} ({list(r.iters)})"
+    assert r.ittot == ittot, f"{key} n={n}: {r.ittot} iterations, published {ittot} ({list(r.iters)})"

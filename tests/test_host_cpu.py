@@ -154,3 +154,95 @@ def test_batch_workload_draw(pkg):
         assert m.n in (B.SIZES_2D if nd == 2 else B.SIZES_3D)
         assert 0.5 <= m.amp < 2.0 and len(m.centre) == nd and all(0.3 <= c < 0.7 for c in m.centre)
     assert B.SIZES_2D == list(range(128, 1025, 64)) and B.SIZES_3D == list(range(64, 257, 32))
+
+
+# Rows of the reference's own output files (data): the converging scripts' `nx [ny [nz]] ittot nt`
+# (output/out_diff_1D_lin.txt:1, out_diff_2D_lin.txt:1, out_diff_3D_lin.txt:1) and the perf scripts'
+# rows with T_eff (output/out_diff_2D_nonlin_perf_ampere.txt:1, out_diff_3D_nonlin_perf_ampere.txt:1,
+# out_diff_3D_nonlin_multixpu_perf_ampere.txt:1).
+REFERENCE_ROWS = {
+    1: "32 200 5",
+    2: "32 32 220 5",
+    3: "62 62 62 430 5",
+    "2D_perf": "32 32 200 0.003325939178466797 6.5536e-5 3.325939178466797e-5 1.9704509458351256",
+    "3D_perf": "1 32 32 32 2000 0.038771867752075195 0.00262144 3.87718677520752e-5 67.61190914924886",
+    "3D_multixpu_perf": "1 512 512 512 400 2.2332510948181152 10.73741824 0.011816143358826007 908.7075125895237",
+}
+
+
+def _shape(tok):
+    return ["i" if re.fullmatch(r"-?\d+", t) else "f" for t in tok.split()]
+
+
+def test_julia_float_reproduces_reference_rows():
+    """julia_float writes every float of the reference's rows exactly as the file holds it."""
+    from importlib import import_module
+    S = import_module("ptd_amd.scripts")
+    for row in REFERENCE_ROWS.values():
+        for tok in row.split():
+            if "." in tok or "e" in tok:
+                assert S.julia_float(float(tok)) == tok, tok
+
+
+@pytest.mark.parametrize("name", ["diff_1D_lin", "diff_1D_linstep", "diff_1D_nonlin", "diff_2D_lin",
+                                  "diff_2D_linstep", "diff_2D_nonlin", "diff_2D_nonlin_perf", "diff_3D_lin_multixpu",
+                                  "diff_3D_linstep_multixpu", "diff_3D_nonlin_multixpu", "diff_3D_nonlin_perf",
+                                  "diff_3D_nonlin_multixpu_perf", "diff_1D_react", "diff_2D_react"])
+def test_main_output_rows(name, tmp_path, monkeypatch, capsys):
+    """main() with DO_SAVE=1 on a stubbed GPU run: the row lands in the file the reference appends to
+    (the multi-xPU 3D converging scripts without their suffix), with the reference row's fields in order;
+    the summary lines are the scripts' printf lines (the 1D linear one with its analytic error); the
+    reaction scripts write no row."""
+    from importlib import import_module
+    S = import_module("ptd_amd.scripts")
+    spec = S.SCRIPTS[name]
+    nd = spec["ndim"]
+    n = {1: [32], 2: [32, 32], 3: [32, 32, 32]}[nd]
+
+    def fake_run(nm, nx, ny=None, nz=None, **kw):
+        coords = tuple(S.linrange(10.0 / v / 2, 10.0 - 10.0 / v / 2, v) for v in n)
+        H = np.asfortranarray(np.full(tuple(n), 0.5))
+        iters = np.array([40, 30, 30, 30, 30][: 1 if spec.get("react") else (2 if spec.get("perf") else 5)])
+        r = S.ScriptResult(nm, coords, H, iters, np.full(len(iters), 9e-9))
+        if spec.get("perf"):
+            r.t_toc, r.niter = 0.0123, int(iters[-1])
+            r.A_eff = S.a_eff_gb(nd, n)
+            r.t_it = r.t_toc / r.niter
+            r.T_eff = r.A_eff / r.t_it
+        return r
+
+    monkeypatch.setattr(S, "run_script", fake_run)
+    monkeypatch.chdir(tmp_path)
+    for k, v in zip(("NX", "NY", "NZ"), n):
+        monkeypatch.setenv(k, str(v))
+    monkeypatch.setenv("DO_SAVE", "true")
+    assert S.main([name]) == 0
+    printed = capsys.readouterr().out.splitlines()
+    files = sorted(p.name for p in (tmp_path / "output").glob("*")) if (tmp_path / "output").exists() else []
+    if spec.get("react"):
+        assert files == []
+        assert printed[-1].startswith("nx = 32, ")
+        return
+    expect = {"diff_3D_lin_multixpu": "out_diff_3D_lin.txt", "diff_3D_linstep_multixpu": "out_diff_3D_linstep.txt",
+              "diff_3D_nonlin_multixpu": "out_diff_3D_nonlin.txt"}.get(name, f"out_{name}.txt")
+    assert files == [expect]
+    row = (tmp_path / "output" / expect).read_text().splitlines()
+    assert len(row) == 1
+    ref = REFERENCE_ROWS["2D_perf" if name == "diff_2D_nonlin_perf" else "3D_perf" if name == "diff_3D_nonlin_perf"
+                         else "3D_multixpu_perf" if name == "diff_3D_nonlin_multixpu_perf" else nd]
+    assert _shape(row[0]) == _shape(ref), (row[0], ref)
+    toks = row[0].split()
+    if spec.get("perf"):
+        assert toks[: len(toks) - 5][-nd:] == [str(v) for v in n]
+    else:
+        assert toks == [str(v) for v in n] + ["160", "5"]
+    total = [ln for ln in printed if ln.startswith("Total time = ")]
+    assert len(total) == 1 and total[0].endswith(" ")
+    if spec.get("perf"):
+        assert printed[0].startswith("PERF: Time = 0.012 sec, T_eff = ")
+    if name == "diff_1D_lin":
+        m = re.search(r"error vs analytic = ([0-9.e+-]+) $", total[0])
+        assert m, total[0]
+        xc = S.linrange(10.0 / 64, 10.0 - 10.0 / 64, 32)
+        hana = 1 / np.sqrt(4 * 1.25) * np.exp(-(xc - 5.0) ** 2 / (4 * 1.25))
+        assert m.group(1) == "%1.2e" % (np.linalg.norm(0.5 - hana) / np.sqrt(32))

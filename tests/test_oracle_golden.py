@@ -92,3 +92,15 @@ def test_fixture_comparison_detects_wrong_maxloc(orc, reftests):
     Hs, _ = sample(H, n, 3)
     rel = np.linalg.norm(Hs - reftests[fixture]["H"]) / np.linalg.norm(reftests[fixture]["H"])
     assert rel > 1e-8
+
+
+@pytest.mark.parametrize("key", ["1D_lin", "1D_linstep", "1D_nonlin"])
+def test_oracle_1d_16384_counts(key, orc):
+    """The oracle's per-step counts at n = 16384 (the values the GPU test asserts for the same inputs). The
+    reference's published totals are not reproduced at this size: the residual hovers within a few percent
+    of tol for hundreds of iterations, so one-ulp differences in H0 (a different exp) move the stopping
+    check by several nout intervals. Every smaller published size is reproduced exactly."""
+    from conftest import ORACLE_1D_16384
+    script = f"diff_1D_{key.split('_', 1)[1]}"
+    r = orc.run_script(script, 16384)
+    assert [int(v) for v in r["iters"]] == ORACLE_1D_16384[key]
