@@ -177,6 +177,33 @@ int32_t ptd_pt_solve(ptd_ctx* ctx, int64_t* iters, double* err_rms, double* err_
  * iters[nt] and errs[nt] (may be NULL) receive per-step iteration counts and final errors. */
 int32_t ptd_solve(ptd_ctx* ctx, int32_t nt, int64_t* iters, double* errs);
 
+/* ---- a PT solve that allocates nothing ----------------------------------------------------------
+ *
+ * No counterpart in the reference. For callers that cannot allocate device memory (code under HIP graph
+ * capture, tight serving loops, harnesses that forbid hipMalloc): ptd_pt_solve on caller-owned device
+ * buffers, with no context. The residual reduction's scratch (per-tile partials of the checking sweeps
+ * and the reduced sum) lives in a fixed library array of PTD_NOALLOC_MAX_PARTIALS + 1 (sum, max) pairs
+ * (2 MiB) per device, held until the process ends; grids whose checks need more partials (cubes
+ * beyond about 1260^3, squares beyond about 10600^2) are refused with PTD_ERR_ARG. The call makes no
+ * allocation (no hipMalloc / hipHostMalloc / hipFree). Calls that overlap on one device share that
+ * array and race: serialise them (one stream, or one host thread per device).
+ *
+ *  params   : ndim 2 or 3, form PTD_PLAIN, precision PTD_FP64 or PTD_FP32 (the device buffers hold
+ *             double or float accordingly), arith as for contexts.
+ *  stream   : hipStream_t to run on (NULL: the null stream); the call returns after it has synchronised.
+ *  H0, H1   : the two H buffers (nx*ny*nz each); BOTH must hold the Dirichlet boundary values, which no
+ *             sweep writes. Hold (nx*ny*nz), D (the step case, nx*ny*nz; NULL otherwise).
+ *  qx0..qz1 : the two buffers of each flux array, reference shapes (qz NULL in 2D).
+ *  parity   : in: which buffer (0 or 1) of H and of the fluxes holds the current iterate; out: the
+ *             buffer holding the result.
+ * Iteration counts, errors and results are bit-identical to ptd_pt_solve on a context holding the same
+ * fields. */
+#define PTD_NOALLOC_MAX_PARTIALS 131072
+int32_t ptd_pt_solve_noalloc(const ptd_params* params, int32_t device, void* stream, void* H0, void* H1,
+                             const void* Hold, const void* D, void* qx0, void* qx1, void* qy0, void* qy1,
+                             void* qz0, void* qz1, int32_t* parity, int64_t* iters, double* err_rms,
+                             double* err_max);
+
 /* ---- gradients (lin / step cases) ----------------------------------------------------------------
  *
  * No reference counterpart: the reference has no derivative. Runs ptd_solve(nt) from the context's

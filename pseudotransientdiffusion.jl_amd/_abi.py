@@ -19,6 +19,7 @@ FP64, FP32, MIXED = 0, 1, 2
 NORM_RMS, NORM_MAX = 0, 1
 ARITH_EXACT, ARITH_FAST = 0, 1
 F_H, F_HOLD, F_D, F_QX, F_QY, F_QZ, F_QX2, F_QY2, F_QZ2, F_RESH, F_DTAU = range(11)
+NOALLOC_MAX_PARTIALS = 131072   # PTD_NOALLOC_MAX_PARTIALS
 
 STATUS_NAMES = {OK: "PTD_OK", ERR_ARG: "PTD_ERR_ARG", ERR_HIP: "PTD_ERR_HIP", ERR_NAN: "PTD_ERR_NAN",
                 ERR_NOMEM: "PTD_ERR_NOMEM", ERR_STATE: "PTD_ERR_STATE", ERR_NODEV: "PTD_ERR_NODEV"}
@@ -32,7 +33,7 @@ EXPORTS = (
     "ptd_compute_iter_params", "ptd_compute_flux", "ptd_compute_update", "ptd_compute_flux_res",
     "ptd_check_res", "ptd_assign",
     "ptd_iterate", "ptd_iterate_timed", "ptd_pt_solve", "ptd_solve",
-    "ptd_grad_solve",
+    "ptd_grad_solve", "ptd_pt_solve_noalloc",
     "ptd_batch_create", "ptd_batch_destroy", "ptd_batch_size", "ptd_batch_device_bytes",
     "ptd_batch_set_field", "ptd_batch_get_field", "ptd_batch_solve", "ptd_batch_tiling",
 )
@@ -99,6 +100,8 @@ def load():
         "ptd_pt_solve": (i32, [vp, i64p, dp, dp]),
         "ptd_solve": (i32, [vp, i32, i64p, dp]),
         "ptd_grad_solve": (i32, [vp, i32, dp, i64, dp, dp, dp, i64p, i64p]),
+        "ptd_pt_solve_noalloc": (i32, [pp, i32, vp, vp, vp, vp, vp, vp, vp, vp, vp, vp, vp,
+                                       ctypes.POINTER(i32), i64p, dp, dp]),
         "ptd_batch_create": (i32, [pp, i32, i32, ctypes.POINTER(vp)]),
         "ptd_batch_destroy": (i32, [vp]),
         "ptd_batch_size": (i32, [vp]),

@@ -7,6 +7,7 @@
 #include <cstdio>
 #include <cstdlib>
 #include <cstring>
+#include <mutex>
 #include <string>
 #include <vector>
 
@@ -184,13 +185,16 @@ int device_guard(ptd_ctx* c) {
     return PTD_OK;
 }
 
-// Reads (sum r^2, max|r|) of the last reduction and forms the two norms.
+// Reads (sum r^2, max|r|) of the last reduction and forms the two norms (into the context's pinned
+// mirror, or a stack buffer for contexts without one).
 int fetch_errors(ptd_ctx* c, double* rms, double* mx) {
-    PTD_HIP(hipMemcpyAsync(c->hred, c->dred, 2 * sizeof(double), hipMemcpyDeviceToHost, c->stream));
+    double local[2] = {0.0, 0.0};
+    double* h = c->hred ? c->hred : local;
+    PTD_HIP(hipMemcpyAsync(h, c->dred, 2 * sizeof(double), hipMemcpyDeviceToHost, c->stream));
     PTD_HIP(hipStreamSynchronize(c->stream));
     const double len = c->p.norm_len > 0 ? c->p.norm_len : (double)c->nR;
-    *rms = std::sqrt(c->hred[0]) / std::sqrt(len);
-    *mx = c->hred[1];
+    *rms = std::sqrt(h[0]) / std::sqrt(len);
+    *mx = h[1];
     return PTD_OK;
 }
 
@@ -369,6 +373,61 @@ int pt_solve_mixed(ptd_ctx* c, int64_t* iters, double* err_rms, double* err_max)
     if (err_rms) *err_rms = rms;
     if (err_max) *err_max = mx;
     if (std::isnan(err)) return fail(PTD_ERR_NAN, "NaN");
+    return PTD_OK;
+}
+
+// One PT solve at fixed Hold (float64 / float32 sweeps): `while err > tol && iter < itmax` with a check
+// every nout iterations (scripts/diff_3D/diff_3D_nonlin_multixpu.jl:102-115). err only changes at checks,
+// so the iterations up to the next check (or itMax) run as one block.
+int pt_solve_loop(ptd_ctx* c, int64_t* iters, double* err_rms, double* err_max) {
+    const ptd_params& p = c->p;
+    int64_t iter = 0;
+    double err = 2.0 * p.tol, rms = err, mx = err;
+    while (err > p.tol && iter < p.itmax) {
+        const int64_t next_check = (iter / p.nout + 1) * (int64_t)p.nout;
+        const int64_t stop = next_check < p.itmax ? next_check : p.itmax;
+        const bool check = (stop % p.nout) == 0;
+        int st = run_iterations(c, stop - iter, check);
+        if (st != PTD_OK) return st;
+        iter = stop;
+        if (check) {
+            st = fetch_errors(c, &rms, &mx);
+            if (st != PTD_OK) return st;
+            err = p.stop_norm == PTD_NORM_MAX ? mx : rms;
+        }
+    }
+    PTD_HIP(hipStreamSynchronize(c->stream));
+    if (iters) *iters = iter;
+    if (err_rms) *err_rms = rms;
+    if (err_max) *err_max = mx;
+    if (std::isnan(err)) return fail(PTD_ERR_NAN, "NaN");
+    return PTD_OK;
+}
+
+// ---- the solve without allocation (ptd_pt_solve_noalloc) --------------------------------------
+//
+// Scratch of the residual reduction: per-tile (sum r^2, max|r|) partials of the checking launch and the
+// reduced pair after them. Every device holds its own copy until the process ends; its device address is
+// looked up once per device and kept in g_scratch (the array's name is a host-side stand-in, never passed
+// to a kernel). Nothing in it needs clearing: every tile of a checking launch writes its partial before
+// the reduction reads it. Calls that overlap on one device share it and race.
+constexpr int64_t kNoAllocPartials = PTD_NOALLOC_MAX_PARTIALS;
+__device__ __align__(256) double2 scratch[kNoAllocPartials + 1];
+
+constexpr int kMaxDevices = 64;
+std::mutex g_scratch_mu;
+double2* g_scratch[kMaxDevices] = {};
+
+int scratch_for(int device, double2** out) {
+    std::lock_guard<std::mutex> lock(g_scratch_mu);
+    if (!g_scratch[device]) {
+        void* ptr = nullptr;
+        const hipError_t e = hipGetSymbolAddress(&ptr, HIP_SYMBOL(scratch));
+        if (e != hipSuccess || !ptr)
+            return fail(PTD_ERR_HIP, std::string("hipGetSymbolAddress(scratch): ") + hipGetErrorString(e));
+        g_scratch[device] = (double2*)ptr;
+    }
+    *out = g_scratch[device];
     return PTD_OK;
 }
 
@@ -614,30 +673,58 @@ int32_t ptd_pt_solve(ptd_ctx* ctx, int64_t* iters, double* err_rms, double* err_
     int st = device_guard(ctx);
     if (st != PTD_OK) return st;
     if (ctx->mixed) return pt_solve_mixed(ctx, iters, err_rms, err_max);
-    const ptd_params& p = ctx->p;
-    int64_t iter = 0;
-    double err = 2.0 * p.tol, rms = err, mx = err;
-    // while err > tol && iter < itMax (scripts/diff_3D/diff_3D_nonlin_multixpu.jl:104-115). err only
-    // changes at checks, so the iterations up to the next check (or itMax) run as one block.
-    while (err > p.tol && iter < p.itmax) {
-        const int64_t next_check = (iter / p.nout + 1) * (int64_t)p.nout;
-        const int64_t stop = next_check < p.itmax ? next_check : p.itmax;
-        const bool check = (stop % p.nout) == 0;
-        st = run_iterations(ctx, stop - iter, check);
-        if (st != PTD_OK) return st;
-        iter = stop;
-        if (check) {
-            st = fetch_errors(ctx, &rms, &mx);
-            if (st != PTD_OK) return st;
-            err = p.stop_norm == PTD_NORM_MAX ? mx : rms;
-        }
-    }
-    PTD_HIP(hipStreamSynchronize(ctx->stream));
-    if (iters) *iters = iter;
-    if (err_rms) *err_rms = rms;
-    if (err_max) *err_max = mx;
-    if (std::isnan(err)) return fail(PTD_ERR_NAN, "NaN");
-    return PTD_OK;
+    return pt_solve_loop(ctx, iters, err_rms, err_max);
+}
+
+int32_t ptd_pt_solve_noalloc(const ptd_params* params, int32_t device, void* stream, void* H0, void* H1,
+                             const void* Hold, const void* D, void* qx0, void* qx1, void* qy0, void* qy1,
+                             void* qz0, void* qz1, int32_t* parity, int64_t* iters, double* err_rms,
+                             double* err_max) {
+    int st = check_params(params);
+    if (st != PTD_OK) return st;
+    PTD_REQUIRE(params->ndim >= 2, PTD_ERR_ARG, "ptd_pt_solve_noalloc runs the fused 2D / 3D sweeps: ndim must be 2 or 3");
+    PTD_REQUIRE(params->form == PTD_PLAIN, PTD_ERR_ARG,
+                "ptd_pt_solve_noalloc checks inside the fused sweep: form must be PTD_PLAIN");
+    PTD_REQUIRE(params->precision == PTD_FP64 || params->precision == PTD_FP32, PTD_ERR_ARG,
+                "ptd_pt_solve_noalloc: precision must be PTD_FP64 or PTD_FP32");
+    PTD_REQUIRE(parity != nullptr && (*parity == 0 || *parity == 1), PTD_ERR_ARG, "parity must point to 0 or 1");
+    PTD_REQUIRE(H0 && H1 && Hold && qx0 && qx1 && qy0 && qy1, PTD_ERR_ARG, "a field pointer is NULL");
+    PTD_REQUIRE(params->ndim < 3 || (qz0 && qz1), PTD_ERR_ARG, "qz buffers are NULL");
+    PTD_REQUIRE(params->dcase != PTD_STEP || D, PTD_ERR_ARG, "the step case needs D");
+    PTD_REQUIRE(device >= 0 && device < kMaxDevices, PTD_ERR_ARG, "device index out of range");
+    st = ptd_device_check(device, nullptr, 0);
+    if (st != PTD_OK) return st;
+    PTD_HIP(hipSetDevice(device));
+    double2* scr = nullptr;
+    st = scratch_for(device, &scr);
+    if (st != PTD_OK) return st;
+    ptd_ctx c;
+    std::memset((void*)&c, 0, sizeof(c));
+    c.p = *params;
+    if (c.p.ndim < 3) c.p.nz = 1;
+    c.device = device;
+    c.stream = (hipStream_t)stream;
+    c.esz = params->precision == PTD_FP32 ? 4 : 8;
+    shapes(c.p, c.nH, c.nQ, c.nR);
+    PTD_REQUIRE(ptd::fused_available(&c), PTD_ERR_ARG, "grid not supported by the fused sweeps");
+    const int64_t need = ptd::fused_check_tiles(&c);
+    PTD_REQUIRE(need <= kNoAllocPartials, PTD_ERR_ARG,
+                "grid too large for ptd_pt_solve_noalloc: its checks reduce " + std::to_string(need) +
+                    " tile partials, the scratch holds " + std::to_string(kNoAllocPartials));
+    c.H[0] = H0;
+    c.H[1] = H1;
+    void* q[3][2] = {{qx0, qx1}, {qy0, qy1}, {qz0, qz1}};
+    for (int a = 0; a < 3; ++a)
+        for (int b = 0; b < 2; ++b) c.q[a][b] = c.p.ndim == 2 && a == 2 ? nullptr : q[a][b];
+    c.hcur = c.qcur = *parity;
+    c.Hold = const_cast<void*>(Hold);
+    c.D = const_cast<void*>(D);
+    c.partial = (double*)scr;
+    c.npartial = kNoAllocPartials;
+    c.dred = (double*)(scr + kNoAllocPartials);
+    st = pt_solve_loop(&c, iters, err_rms, err_max);
+    *parity = c.hcur;
+    return st;
 }
 
 int32_t ptd_solve(ptd_ctx* ctx, int32_t nt, int64_t* iters, double* errs) {

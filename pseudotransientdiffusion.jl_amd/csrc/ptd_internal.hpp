@@ -486,6 +486,8 @@ hipError_t launch_fused2(ptd_ctx* c, bool check);         // two fused PT iterat
 int64_t max_chunk_tz(const ptd_params& p, int64_t esz);
 bool fused_available(const ptd_ctx* c);
 bool fused2_available(const ptd_ctx* c);
+// residual partials (tiles) the checking launches of a fused 2D / 3D solve write at most
+int64_t fused_check_tiles(const ptd_ctx* c);
 // mixed precision: float32 correction sweeps (1 or 2 iterations), source and correction kernels
 hipError_t launch_delta_sweep(ptd_ctx* c, bool two, bool check);
 hipError_t launch_f64_to_f32(ptd_ctx* c, float* dst, const double* src, int64_t n);

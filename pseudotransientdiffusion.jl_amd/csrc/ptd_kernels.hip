@@ -1753,6 +1753,23 @@ hipError_t launch_delta_sweep(ptd_ctx* c, bool two, bool check) {
 }
 
 // Number of (sum,max) partials the reductions can produce for this context.
+// Residual partials the checking launches of a fused float64 / float32 PT solve (2D / 3D) write: the
+// largest tile count over the single-level and two-level checking tilings (the two-level one counted with
+// its smallest block, 12 rows), at the z chunk depth the checking launches use.
+int64_t fused_check_tiles(const ptd_ctx* c) {
+    const ptd_params& p = c->p;
+    if (p.ndim == 2) {
+        const int64_t ntx = (p.nx - 2 + 61) / 62, nty = (p.ny - 2 + kRows2D - 3) / (kRows2D - 2);
+        const int64_t ntx2 = (p.nx - 2 + 59) / 60, nty2 = (p.ny - 2 + kTy2D - 1) / kTy2D;
+        return std::max<int64_t>(ntx * nty, ntx2 * nty2);
+    }
+    const int64_t tz = fused_tz(c, true);
+    const int64_t nbz = (p.nz - 2 + tz - 1) / tz;
+    const int64_t ntx = (p.nx - 2 + 61) / 62, nty = (p.ny - 2 + kRows - 3) / (kRows - 2);
+    const int64_t ntx2 = (p.nx - 2 + 59) / 60, nty2 = (p.ny - 2 + kMinRows2 - 5) / (kMinRows2 - 4);
+    return std::max<int64_t>(ntx * nty, ntx2 * nty2) * nbz;
+}
+
 int64_t partial_capacity(const ptd_params& p, int64_t nR) {
     int64_t nb = (nR + kBlock - 1) / kBlock;
     if (p.ndim == 2) {
