@@ -8,10 +8,19 @@ value = T_eff = A_eff / t_it with A_eff = (2*4+2)*nx*ny*nz*8 bytes per iteration
 definition (scripts/diff_3D/diff_3D_nonlin_perf.jl:114-116). K iterations are timed with HIP events on
 the library's stream, bracketed by device synchronisation.
 
+roofline: the dominant kernel is the two-level sweep (one launch = two PT iterations, one pass over the
+fields). achieved = its compulsory HBM bytes per launch, (N + 2r + 6q)*8 (read H over the N cells, Hold
+and write H over the r inner cells, read and write the three flux arrays of q faces each), divided by the
+launch time; frac = achieved / 8 TB/s. traffic = the launch's measured HBM bytes (rocprofv3 FETCH_SIZE +
+WRITE_SIZE, committed profile), traffic_frac = traffic / launch time / 8 TB/s. teff_over_peak = T_eff /
+8 TB/s, which counts the reference's A_eff for each of the launch's two iterations and can exceed 1.
+
 Extra legs (N=1), each in its own object of the JSON line:
  - time_to_solution: the full converging 5-step solve (tol 1e-8, check every nout=10 iterations) in
-   float64 and with float32 correction sweeps inside float64 defect correction (--tts-n 1023 for the
-   precision workload's grid);
+   float64 and with float32 correction sweeps inside float64 defect correction, on the precision
+   workload's grid (1023^3); time_to_solution_511 the same at 511^3;
+ - step_half_x: the configuration's step case at 511^3 (D = 1e-4 where x < lx/2, 1 elsewhere, same IC):
+   its sweep's T_eff and its converging 5-step float64 solve;
  - batch: 96 independent 2D / 3D solves of uneven sizes and D types in one batch (aggregate T_eff,
    p50 / p99 latency);
  - gradient: dL/dD of the 3D step case at 255^3 over 10 steps by the adjoint, checked by central
@@ -40,6 +49,26 @@ def log(msg):
 
 
 TRAFFIC_PROFILE = os.path.join(ROOT, "profiles", "fused3d_t2_nonlin_511_traffic.md")
+
+
+def cpu_model():
+    """The host CPU's model name and logical core count (reported beside the CPU baseline)."""
+    name = "unknown"
+    try:
+        for ln in open("/proc/cpuinfo"):
+            if ln.startswith("model name"):
+                name = ln.split(":", 1)[1].strip()
+                break
+    except OSError:
+        pass
+    return name, os.cpu_count() or 0
+
+
+def compulsory_bytes(n, esz):
+    """HBM bytes one pass of the fused sweep cannot avoid on an n^3 grid: read H (N cells), read Hold and
+    write H on the r = (n-2)^3 inner cells, read and write qx, qy, qz (q = (n-1)(n-2)^2 faces each)."""
+    N, r, q = n ** 3, (n - 2) ** 3, (n - 1) * (n - 2) ** 2
+    return (N + 2 * r + 6 * q) * esz
 
 
 def calib_copy():
@@ -173,6 +202,84 @@ def grad_leg(pkg, n, nfd, seed=0, nt=10):
             "fd_rel_err_median": float(np.median(errs)) if errs else None, "tol": p.tol}
 
 
+def tts_leg(pkg, tn, arith, sync):
+    """Time to solution on the 3D nonlinear case at tn^3 (benchmark IC): the converging 5-step solve
+    (scripts/diff_3D/diff_3D_nonlin_multixpu.jl parameters: tol 1e-8 on the RMS residual, nout 10) in
+    float64, then with float32 correction sweeps inside float64 defect correction (PTD_MIXED), and the
+    relative max-norm difference of their final H."""
+    import numpy as np
+
+    spec = pkg.SCRIPTS["diff_3D_nonlin_perf"]
+    sp = dict(pkg.SCRIPTS["diff_3D_nonlin_multixpu"])
+    log(f"building config IC at {tn}^3")
+    H0 = pkg.initial_condition(spec, tn, tn, tn, kind="config", amp=2.0, bval=1.0)
+    nt = int(math.ceil(sp["ttot"] / sp["dt"]))
+    out = {"n": tn, "tol": sp["tol"], "nt": nt}
+    H64 = None
+    for label, prec in (("fp64", pkg.FP64), ("mixed", pkg.MIXED)):
+        p = pkg.make_params(sp, tn, tn, tn, precision=prec, arith=arith)
+        with pkg.Context(p) as ctx:
+            ctx.set("H", H0)
+            ctx.set("Hold", H0)
+            sync()
+            log(f"time-to-solution {label} at {tn}^3 (5 steps, tol 1e-8)")
+            t0 = time.perf_counter()
+            iters, errs = solve_steps(ctx, nt, label)
+            sync()
+            el = time.perf_counter() - t0
+            H = ctx.get("H")
+        out[f"{label}_s"] = round(el, 4)
+        out[f"{label}_iters"] = [int(v) for v in iters]
+        out[f"{label}_final_err"] = float(errs[-1])
+        if label == "fp64":
+            H64 = H
+    out["speedup"] = round(out["fp64_s"] / out["mixed_s"], 3)
+    out["mixed_vs_fp64_rel_maxnorm"] = float(np.max(np.abs(H - H64)) / np.max(np.abs(H64)))
+    log(f"time-to-solution at {tn}^3: fp64 {out['fp64_s']:.3f} s, mixed {out['mixed_s']:.3f} s "
+        f"({out['speedup']:.3f}x), max-norm difference {out['mixed_vs_fp64_rel_maxnorm']:.2e}")
+    return out
+
+
+def step_leg(pkg, n, steps, warmup, sync):
+    """The configuration's step case at n^3: D = 1e-4 where the cell centre x < lx/2, 1 elsewhere, with
+    the benchmark IC (H0 = 1 + 2 exp(-r^2), H = 1 on the boundary) and the linstep script's numerics
+    (scripts/diff_3D/diff_3D_linstep_multixpu.jl:67-108). Reports the FAST and EXACT sweeps' t_it / T_eff
+    (events over `steps` iterations after `warmup`) and the converging 5-step float64 solve."""
+    import numpy as np
+
+    sp = pkg.SCRIPTS["diff_3D_linstep_multixpu"]
+    H0 = pkg.initial_condition(sp, n, n, n, kind="config", amp=2.0, bval=1.0)
+    D = pkg.step_field(sp, n, n, n, kind="half_x")
+    a_eff = pkg.a_eff_gb(3, (n, n, n), 8)
+    out = {"workload": f"3D step D (1e-4 for x < lx/2, 1 elsewhere), {n}^3, H0=1+2exp(-r^2), H=1 boundaries",
+           "n": n}
+    for label, arith in (("fast", 1), ("exact", 0)):
+        p = pkg.make_params(sp, n, n, n, arith=arith)
+        with pkg.Context(p) as ctx:
+            ctx.set("H", H0)
+            ctx.set("Hold", H0)
+            ctx.set("D", D)
+            ctx.iterate(warmup)
+            ms = ctx.iterate_timed(steps) / steps
+            out[f"{label}_t_it_ms"] = round(ms, 5)
+            out[f"{label}_T_eff"] = round(a_eff / (ms / 1e3), 2)
+            if arith == 1:
+                ctx.set("H", H0)
+                ctx.set("Hold", H0)
+                for f in ("qHx", "qHy", "qHz"):  # the script's q arrays start at zero (@zeros)
+                    ctx.set(f, np.zeros(ctx.get(f).shape, order="F"))
+                sync()
+                t0 = time.perf_counter()
+                iters, errs = ctx.solve(5)
+                sync()
+                out["solve_s"] = round(time.perf_counter() - t0, 4)
+                out["solve_iters"] = [int(v) for v in iters]
+                out["solve_final_err"] = float(errs[-1])
+    log(f"step case {n}^3: FAST t_it {out['fast_t_it_ms']} ms (T_eff {out['fast_T_eff']} GB/s), EXACT "
+        f"{out['exact_t_it_ms']} ms; 5-step solve {out['solve_s']} s, iterations {out['solve_iters']}")
+    return out
+
+
 def cpu_baseline(pkg, n, seconds):
     """The oracle's PT iterations (OpenMP C restatement of the reference Threads path) at n^3 on this
     host: run iterations until `seconds` have elapsed (at least one), report T_eff."""
@@ -199,7 +306,9 @@ def cpu_baseline(pkg, n, seconds):
     t_it = el / it
     a_eff = pkg.a_eff_gb(3, (n, n, n))
     threads = int(os.environ.get("OMP_NUM_THREADS", os.cpu_count() or 1))
+    model, logical = cpu_model()
     return {"value": round(a_eff / t_it, 3), "unit": "GB/s", "cores": threads, "kind": "port",
+            "cpu_model": model, "host_logical_cores": logical,
             "sample": f"{it} PT iterations of the oracle (C/OpenMP restatement) at {n}^3 float64, "
                       f"{el:.1f} s, t_it = {t_it * 1e3:.1f} ms"}
 
@@ -276,8 +385,12 @@ def main():
     ap.add_argument("--repeats", type=int, default=5,
                     help="repeats of the K timed iterations for the median / max t_it (0: none)")
     ap.add_argument("--cpu-seconds", type=float, default=15.0)
-    ap.add_argument("--tts-n", type=int, default=0,
-                    help="grid size of the time-to-solution legs (default: --n; the precision workload is 1023)")
+    ap.add_argument("--tts-n", type=str, default="1023,511",
+                    help="grid sizes of the time-to-solution legs, comma separated (the first is the precision "
+                         "workload's grid, reported as time_to_solution; the others as time_to_solution_<n>); "
+                         "0 skips them")
+    ap.add_argument("--step-n", type=int, default=511,
+                    help="grid of the step-case leg (D = 1e-4 for x < lx/2); 0 skips it")
     ap.add_argument("--batch", type=int, default=96,
                     help="members of the batched workload leg (2D {128..1024}^2 and 3D {64..256}^3 grids, "
                          "lin / step / H^3, 5 physical steps each); 0 skips it")
@@ -352,6 +465,10 @@ def main():
     # (achieved = 2*A_eff / (2*t_it) = T_eff)
     iters_per_launch = 2 if (args.steps % 2 == 0 and os.environ.get("PTD_T2", "1") != "0") else 1
     traffic = measured_traffic(n, "f64" if prec == pkg.FP64 else "f32", args.arith)
+    # roofline of the dominant kernel: its compulsory bytes per launch over the launch time
+    launch_s = t_it * iters_per_launch
+    comp = compulsory_bytes(n, esz)
+    achieved = comp / launch_s / 1e9
     log(f"t_it = {t_it * 1e3:.4f} ms (events), wall {wall / args.steps * 1e3:.4f} ms, T_eff = {t_eff:.1f} GB/s")
 
     out = {
@@ -373,15 +490,16 @@ def main():
                    "kernel": "k_fused3d_t2" if iters_per_launch == 2 else "k_fused3d",
                    "iterations_per_launch": iters_per_launch,
                    "launch_us": round(t_it * iters_per_launch * 1e6, 2)},
-        "roofline": {"bound": "hbm", "achieved": round(t_eff, 2), "peak": PEAK_HBM_GBS, "unit": "GB/s",
-                     "frac": round(t_eff / PEAK_HBM_GBS, 4), "traffic": traffic},
+        "roofline": {"bound": "hbm", "achieved": round(achieved, 2), "peak": PEAK_HBM_GBS, "unit": "GB/s",
+                     "frac": round(achieved / PEAK_HBM_GBS, 4), "traffic": traffic,
+                     "compulsory_bytes": int(comp), "launch_us": round(launch_s * 1e6, 2),
+                     "teff_over_peak": round(t_eff / PEAK_HBM_GBS, 4)},
     }
     if traffic:
-        # achieved counts the reference's A_eff for the launch's two iterations; the measured HBM bytes
-        # of the launch, at the same launch time, give the actual HBM rate
-        launch_s = t_it * iters_per_launch
+        # the launch's measured HBM bytes at the same launch time: the actual HBM rate
         out["roofline"]["traffic_GBs"] = round(traffic / launch_s / 1e9, 2)
         out["roofline"]["traffic_frac"] = round(traffic / launch_s / 1e9 / PEAK_HBM_GBS, 4)
+        out["roofline"]["traffic_over_compulsory"] = round(traffic / comp, 4)
 
     if rep_ms:
         med = sorted(rep_ms)[len(rep_ms) // 2]
@@ -399,46 +517,15 @@ def main():
     if ms_exact is not None:
         out["exact_arith"] = {"ms_per_step": round(ms_exact, 5), "T_eff": round(a_eff / (ms_exact / 1e3), 2)}
 
-    if not args.no_tts:
-        # time-to-solution: converging solve, 5 physical steps, tol 1e-8 on the RMS residual, nout 10
-        sp = dict(pkg.SCRIPTS["diff_3D_nonlin_multixpu"])
-        tn = args.tts_n or n
-        if tn != n:
-            log(f"building config IC at {tn}^3")
-            H0 = pkg.initial_condition(spec, tn, tn, tn, kind="config", amp=2.0, bval=1.0)
-        p2 = pkg.make_params(sp, tn, tn, tn, precision=pkg.FP64, arith=arith)
-        nt = int(math.ceil(sp["ttot"] / sp["dt"]))
-        with pkg.Context(p2) as ctx:
-            ctx.set("H", H0)
-            ctx.set("Hold", H0)
-            sync()
-            log("time-to-solution FP64 (5 steps, tol 1e-8)")
-            t0 = time.perf_counter()
-            iters, errs = solve_steps(ctx, nt, "fp64")
-            sync()
-            tts = time.perf_counter() - t0
-            H64 = ctx.get("H")
-        out["time_to_solution"] = {"n": tn, "fp64_s": round(tts, 4), "fp64_iters": [int(v) for v in iters],
-                                   "tol": sp["tol"], "nt": nt, "final_err": float(errs[-1])}
-        log(f"time-to-solution {tts:.3f} s, iterations {list(iters)}")
-        # the same solve with float32 correction sweeps inside float64 defect correction
-        p3 = pkg.make_params(sp, tn, tn, tn, precision=pkg.MIXED, arith=arith)
-        with pkg.Context(p3) as ctx:
-            ctx.set("H", H0)
-            ctx.set("Hold", H0)
-            sync()
-            log("time-to-solution FP32 sweeps + FP64 defect correction")
-            t0 = time.perf_counter()
-            miters, merrs = solve_steps(ctx, nt, "mixed")
-            sync()
-            tts_m = time.perf_counter() - t0
-            Hm = ctx.get("H")
-        import numpy as np
-        err_m = float(np.max(np.abs(Hm - H64)) / np.max(np.abs(H64)))
-        out["time_to_solution"].update({"mixed_s": round(tts_m, 4), "mixed_iters": [int(v) for v in miters],
-                                        "mixed_final_err": float(merrs[-1]), "speedup": round(tts / tts_m, 3),
-                                        "mixed_vs_fp64_rel_maxnorm": err_m})
-        log(f"mixed {tts_m:.3f} s ({tts / tts_m:.2f}x), sweeps {list(miters)}, max-norm vs FP64 {err_m:.2e}")
+    tts_sizes = [int(v) for v in str(args.tts_n).split(",") if v.strip() and int(v) > 0]
+    if not args.no_tts and tts_sizes:
+        del H0
+        for k, tn in enumerate(tts_sizes):
+            key = "time_to_solution" if k == 0 else f"time_to_solution_{tn}"
+            out[key] = tts_leg(pkg, tn, arith, sync)
+
+    if args.step_n:
+        out["step_half_x"] = step_leg(pkg, args.step_n, max(20, args.steps // 4), 4, sync)
 
     cfg2d = None
     if not args.no_2d:
