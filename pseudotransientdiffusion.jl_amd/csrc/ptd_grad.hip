@@ -11,6 +11,10 @@
 // (checkpoints); after each adjoint solve one kernel gathers the step's face sensitivities
 // dL/dD_f = (H_b - H_a)(mu_a - mu_b)/h^2 into the cells (half to each side), and the boundary coupling
 // into dL/dH0; dL/dH0 = mu_1/dt on the inner cells. The context ends as after ptd_solve(nt).
+// Each adjoint solve stops at the reduction relative to its right-hand side that the forward solves
+// reach: tol_adj = tol * rms((H^nt - H_obs)*dt) / rms(H^nt) over the inner cells (the forward's right-hand
+// side is Hold ~ H^nt, the adjoint's (H^nt - H_obs)*dt), so a small misfit does not end the adjoint
+// before it starts.
 #include <hip/hip_runtime.h>
 
 #include <cmath>
@@ -62,7 +66,8 @@ extern "C" int32_t ptd_grad_solve(ptd_ctx* c, int32_t nt, const double* h_obs, i
     int status = PTD_OK;
     double *ckpt = nullptr, *hobs = nullptr, *gD = nullptr, *gH0 = nullptr;
     double* qbak[3] = {nullptr, nullptr, nullptr};
-    double red[2] = {0, 0};
+    double red[2] = {0, 0}, hsq[2] = {0, 0};
+    const double tol_fwd = c->p.tol;
     {
         PTD_ALLOCG(&ckpt, (size_t)(nt + 1) * nH * 8);
         PTD_ALLOCG(&hobs, nH * 8);
@@ -90,6 +95,9 @@ extern "C" int32_t ptd_grad_solve(ptd_ctx* c, int32_t nt, const double* h_obs, i
             if (qbak[a])
                 PTD_HIPG(hipMemcpyAsync(qbak[a], c->q[a][c->qcur], (size_t)c->nQ[a] * 8, hipMemcpyDeviceToDevice, c->stream));
 
+        // scale of the forward right-hand side: sum of (H^nt)^2 over the inner cells
+        PTD_HIPG(ptd::launch_sumsq_inner(c, (const double*)c->H[c->hcur]));
+        PTD_HIPG(hipMemcpyAsync(hsq, c->dred, 2 * sizeof(double), hipMemcpyDeviceToHost, c->stream));
         // loss and the adjoint source: Hold = H = (H^nt - H_obs)*dt (0 on the boundary), fluxes from 0
         PTD_HIPG(ptd::launch_adj_source(c, (const double*)c->H[c->hcur], hobs, (double*)c->Hold));
         PTD_HIPG(hipMemcpyAsync(red, c->dred, 2 * sizeof(double), hipMemcpyDeviceToHost, c->stream));
@@ -99,6 +107,9 @@ extern "C" int32_t ptd_grad_solve(ptd_ctx* c, int32_t nt, const double* h_obs, i
                 if (c->q[a][b]) PTD_HIPG(hipMemsetAsync(c->q[a][b], 0, (size_t)c->nQ[a] * 8, c->stream));
         PTD_HIPG(hipStreamSynchronize(c->stream));
         if (loss) *loss = red[0];
+        // rms(r*dt)/rms(H^nt) = sqrt(2L)*dt/sqrt(sum H^2) (both over the inner cells); a zero misfit gives
+        // tol 0 and an adjoint of 0, whose first residual is already 0
+        c->p.tol = hsq[0] > 0.0 ? tol_fwd * std::sqrt(2.0 * red[0]) * c->p.dt / std::sqrt(hsq[0]) : tol_fwd;
 
         // adjoint: mu_nt, ..., mu_1, each followed by its gradient contribution
         for (int n = nt; n >= 1; --n) {
@@ -126,6 +137,7 @@ extern "C" int32_t ptd_grad_solve(ptd_ctx* c, int32_t nt, const double* h_obs, i
         PTD_HIPG(hipStreamSynchronize(c->stream));
     }
 done:
+    c->p.tol = tol_fwd;
     (void)hipStreamSynchronize(c->stream);
     (void)hipFree(ckpt);
     (void)hipFree(hobs);

@@ -505,6 +505,8 @@ hipError_t launch_outer_step(ptd_ctx* c);
 // reduced into c->dred, and one step's gradient contribution
 hipError_t launch_adj_source(ptd_ctx* c, const double* H, const double* Hobs, double* out);
 hipError_t launch_grad_accum(ptd_ctx* c, const double* mu, const double* Hn, double* gD, double* gH0, bool first);
+// sum of H^2 over the inner cells into c->dred
+hipError_t launch_sumsq_inner(ptd_ctx* c, const double* H);
 
 // ---- batches of independent float64 solves (ptd_batch.hip) -----------------------------------
 

@@ -1813,6 +1813,20 @@ k_adj_source(KP<double> p, const double* __restrict__ H, const double* __restric
     block_sum_max(l, 0.0, threadIdx.x, kBlock, &partial[blockIdx.x]);
 }
 
+// Per-block partials of sum H^2 over the inner cells (the scale of a forward solve's right-hand side).
+__global__ void __launch_bounds__(kBlock)
+k_sumsq_inner(KP<double> p, const double* __restrict__ H, double2* __restrict__ partial, int64_t n) {
+    const int64_t c = (int64_t)blockIdx.x * kBlock + threadIdx.x;
+    double l = 0.0;
+    if (c < n) {
+        const int64_t i = c % p.nx, rest = c / p.nx, j = rest % p.ny, k = rest / p.ny;
+        const bool in = i >= 1 && i <= p.nx - 2 && (p.hy == 0 || (j >= 1 && j <= p.ny - 2)) &&
+                        (p.hz == 0 || (k >= 1 && k <= p.nz - 2));
+        l = in ? H[c] * H[c] : 0.0;
+    }
+    block_sum_max(l, 0.0, threadIdx.x, kBlock, &partial[blockIdx.x]);
+}
+
 // Accumulates one step's contribution to the gradient, per cell (gather over the faces at the cell):
 //   gD[c]  += 1/2 sum_{faces f at c} (H_b - H_a)(mu_a - mu_b)/h^2        (face f between a < b)
 //   gH0[c] += sum_{faces f between boundary c and inner i} mu_i D_f/h^2  (c a boundary cell)
@@ -1867,6 +1881,15 @@ hipError_t launch_adj_source(ptd_ctx* c, const double* H, const double* Hobs, do
     const unsigned g = grid_for(c->nH);
     if ((int64_t)g > c->npartial) return hipErrorInvalidValue;
     hipLaunchKernelGGL(k_adj_source, dim3(g), dim3(kBlock), 0, c->stream, kp, H, Hobs, out, (double2*)c->partial, c->nH);
+    hipLaunchKernelGGL(k_reduce_partials, dim3(1), dim3(kBlock), 0, c->stream, (const double2*)c->partial, (int64_t)g,
+                       c->dred);
+    return hipGetLastError();
+}
+
+hipError_t launch_sumsq_inner(ptd_ctx* c, const double* H) {
+    const KP<double> kp = make_kp<double>(c->p);
+    const unsigned g = grid_for(c->nH);
+    hipLaunchKernelGGL(k_sumsq_inner, dim3(g), dim3(kBlock), 0, c->stream, kp, H, (double2*)c->partial, c->nH);
     hipLaunchKernelGGL(k_reduce_partials, dim3(1), dim3(kBlock), 0, c->stream, (const double2*)c->partial, (int64_t)g,
                        c->dred);
     return hipGetLastError();

@@ -86,6 +86,34 @@ def test_gradient_matches_finite_differences_on_gpu(pkg):
         assert abs(fd - gv) <= 1e-5 * abs(gv), (fd, gv)
 
 
+@pytest.mark.parametrize("script,shape", [("diff_3D_linstep_multixpu", (32, 32, 32)), ("diff_2D_lin", (40, 36))])
+def test_gradient_of_a_small_misfit_at_the_default_tol(script, shape, pkg):
+    """H_obs = H(T)*(1 + 1e-6) with the scripts' tol 1e-8: the adjoint right-hand side (H(T) - H_obs)*dt is
+    far below tol in absolute terms, so an adjoint stopped on the forward's absolute tol would end at its
+    first check. The adjoint solves stop relative to their right-hand side instead: the gradient matches
+    the exact adjoint of the same misfit (the reference's H_obs shifted by the exact and GPU H(T)'s
+    difference, so that both sides differentiate the same residual) to 1e-4."""
+    print(f"[gpu] gradient of a 1e-6 misfit {script} {shape}", flush=True)
+    nt = 3
+    p, H0, D, _, h = _setup(pkg, script, shape, tol=1e-8)
+    with pkg.Context(p) as ctx:
+        ctx.set("H", H0)
+        ctx.set("Hold", H0)
+        if p.dcase == pkg.STEP:
+            ctx.set("D", D)
+        ctx.solve(nt)
+        Hg = ctx.get("H")
+    Hobs = np.asfortranarray(Hg * (1.0 + 1e-6))
+    g = _grad_gpu(pkg, p, H0, D, Hobs, nt)
+    assert np.all(g["iters_adj"] > 10)
+    be = BELinear(D, h, p.dt)
+    Hbe = be.forward(H0, nt)[-1]
+    L, gD, gH0 = be.gradient(H0, np.asfortranarray(Hobs + (Hbe - Hg)), nt)
+    assert abs(g["loss"] - L) <= 1e-9 * L
+    assert np.max(np.abs(g["grad_D"] - gD)) <= 1e-4 * np.max(np.abs(gD))
+    assert np.max(np.abs(g["grad_H0"] - gH0)) <= 1e-4 * np.max(np.abs(gH0))
+
+
 def test_context_ends_as_after_solve(pkg):
     script, shape, nt = "diff_3D_linstep_multixpu", (30, 26, 22), 3
     p, H0, D, Hobs, h = _setup(pkg, script, shape, tol=1e-8)
