@@ -104,3 +104,43 @@ def test_mixed_rejects_planes_beyond_the_sweep_offsets(nxy, ok, pkg):
         pkg.Context(p)
     assert e.value.status == abi.ERR_ARG
     assert b"4 GiB" in pkg.load().ptd_last_error()
+
+
+JULIA_DIR = os.path.join(ROOT, "pseudotransientdiffusion.jl_amd", "julia")
+
+
+def test_julia_binding_matches_header(pkg):
+    """The Julia binding (LibPTD.jl; Julia is not installed here, so this is a static check): every ccall
+    names a function include/ptd.h declares, every declared function but the device-buffer entry point is
+    bound, PtdParams has ptd_params' fields in order with the same widths, and the enum values agree."""
+    from importlib import import_module
+    abi = import_module("ptd_amd._abi")
+    src = open(os.path.join(JULIA_DIR, "LibPTD.jl")).read()
+    called = set(re.findall(r"ccall\(\(:(ptd_[a-z0-9_]+),\s*libptd\)", src))
+    declared = set(_declared_functions())
+    assert called <= declared, called - declared
+    assert declared - called == {"ptd_pt_solve_noalloc"}, declared - called
+    body = re.search(r"struct PtdParams\n(.*?)\nend", src, flags=re.S).group(1)
+    fields = re.findall(r"^\s+(\w+)::(\w+)\s*$", body, flags=re.M)
+    jl_width = {"Int32": 4, "Int64": 8, "Float64": 8}
+    assert [(n, jl_width[t]) for n, t in fields] == [(n, ctypes.sizeof(t)) for n, t in abi.Params._fields_]
+    header = open(os.path.join(ROOT, "include", "ptd.h")).read()
+    consts = {}
+    for names, values in re.findall(r"^const ([A-Z0-9_, ]+) = ([\d, ]+)$", src, flags=re.M):
+        consts.update(zip([v.strip() for v in names.split(",")], [int(v) for v in values.split(",")]))
+    enums = re.findall(r"\b(PTD_[A-Z0-9_]+)\s*=\s*(\d+)", header)
+    assert len(enums) >= 30
+    for name, value in enums:
+        assert consts.get(name) == int(value), name
+
+
+def test_julia_script_keeps_the_reference_interface():
+    """The drop-in script keeps the reference script's environment flags, numerics, printed line and row."""
+    src = open(os.path.join(JULIA_DIR, "diff_3D_nonlin_multixpu.jl")).read()
+    for flag in ("USE_RETURN", "DO_SAVE", "NX", "NY", "NZ"):
+        assert f'ENV["{flag}"]' in src
+    for line in ("tol        = 1e-8", "itMax      = 1e5", "nout       = 10", "CFL        = 1 / sqrt(3)",
+                 "Resc       = 1 / 1.2", "dt         = 0.2", "ttot       = 1.0"):
+        assert line in src
+    assert '"Total time = %1.2f, time steps = %d, nx = %d, iterations tot = %d \\n"' in src
+    assert "out_diff_3D_nonlin.txt" in src and "LibPTD.solve!(ctx, nt)" in src
