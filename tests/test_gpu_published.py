@@ -9,6 +9,8 @@ differences in H0 alone (numpy's exp against glibc's, 760 of 16384 cells) move t
 iterations, and FMA contraction by up to 120; the reference's totals come from Julia's exp, CUDA
 arithmetic and CUBLAS nrm2. There the test feeds the oracle's own H0 and requires the oracle's per-step
 counts exactly (pinned on the CPU by test_oracle_golden.py::test_oracle_1d_16384_counts)."""
+import os
+
 import pytest
 
 from conftest import ORACLE_1D_16384
@@ -57,3 +59,38 @@ def test_gpu_published_iteration_counts(key, n, ittot, nt, pkg, orc):
     assert len(r.iters) == nt
     assert all(int(v) % 10 == 0 for v in r.iters)
     assert r.ittot == ittot, f"{key} n={n}: {r.ittot} iterations, published {ittot} ({list(r.iters)})"
+
+
+PUBLISHED_1022 = {"3D_lin": 6080, "3D_linstep": 6000, "3D_nonlin": 2050}  # output/out_diff_3D_*.txt:5
+
+
+@pytest.mark.skipif(os.environ.get("PTD_PUBLISHED_1022") != "1",
+                    reason="about 4 minutes of GPU time (1022^3, EXACT); run with PTD_PUBLISHED_1022=1")
+@pytest.mark.parametrize("key", sorted(PUBLISHED_1022))
+def test_gpu_published_iteration_counts_1022(key, pkg):
+    """The largest published 3D runs (1022^3 on 8 ranks of 512^3, output/out_diff_3D_{lin,linstep,nonlin}.txt:5):
+    the loop-level solve in EXACT arithmetic, with the 8-rank error normalisation len_ResH_g = 1018^3
+    (scripts/diff_3D/diff_3D_nonlin_multixpu.jl:89), needs exactly the published total over 5 steps."""
+    n = 1022
+    case = key.split("_", 1)[1]
+    script = f"diff_3D_{case}_multixpu"
+    spec = pkg.SCRIPTS[script]
+    nloc = (n - 2) // 2 + 2
+    params = pkg.make_params(spec, n, n, n, norm_len=float(((nloc - 4) * 2 + 2) ** 3))
+    print(f"[gpu] published {key} n={n}: building fields", flush=True)
+    H0 = pkg.initial_condition(spec, n, n, n)
+    iters = []
+    with pkg.Context(params) as ctx:
+        ctx.set("H", H0)
+        ctx.set("Hold", H0)
+        del H0
+        if spec["dcase"] == pkg.STEP:
+            ctx.set("D", pkg.step_field(spec, n, n, n))
+            ctx.compute_iter_params()
+        for step in range(5):
+            it, rms, _ = ctx.pt_solve()
+            ctx.assign()
+            iters.append(int(it))
+            print(f"[gpu]   step {step + 1}: {it} iterations, err {rms:.3e}", flush=True)
+    print(f"[gpu] published {key} n={n}: {sum(iters)} (published {PUBLISHED_1022[key]})", flush=True)
+    assert sum(iters) == PUBLISHED_1022[key], iters
