@@ -204,19 +204,24 @@ int32_t ptd_pt_solve_noalloc(const ptd_params* params, int32_t device, void* str
                              void* qz0, void* qz1, int32_t* parity, int64_t* iters, double* err_rms,
                              double* err_max);
 
-/* ---- gradients (lin / step cases) ----------------------------------------------------------------
+/* ---- gradients --------------------------------------------------------------------------------
  *
  * No reference counterpart: the reference has no derivative. Runs ptd_solve(nt) from the context's
- * current state while keeping H^0..H^nt on the device, then the adjoint of the nt backward-Euler steps
- * (for D independent of H the step operator A = I/dt - div(av(D) grad .) is symmetric, so each adjoint
- * solve A mu = r is the same PT solve, with homogeneous boundary values), and returns
+ * current state while keeping H^0..H^nt on the device, then the adjoint of the nt backward-Euler steps,
+ * and returns
  *   *loss    = L = 1/2 sum over the inner cells of (H^nt - H_obs)^2,
- *   grad_d   = dL/dD per cell (nx*ny*nz; in the lin case the sensitivity to a per-cell D at d0),
- *   grad_h0  = dL/dH0 per cell (inner cells: mu_1/dt; boundary cells: their Dirichlet coupling),
+ *   grad_d   = dL/dD per cell (nx*ny*nz; in the lin case the sensitivity to a per-cell D at d0; must be NULL
+ *              for PTD_NONLIN, where D = H^3 is no parameter),
+ *   grad_h0  = dL/dH0 per cell (inner cells: lam_1/dt; boundary cells: their Dirichlet coupling),
  *   iters_fwd[nt], iters_adj[nt] = PT iterations of each forward / adjoint solve.
- * Any output may be NULL. On success the context ends as after ptd_solve(nt); after an error (NaN,
- * allocation) its fields are unspecified and should be set again. Allocates (nt + 4) grid-sized
- * float64 buffers and a copy of the fluxes for the call (PTD_ERR_NOMEM if they do not fit). float64, PTD_LIN or PTD_STEP. */
+ * lin / step: the step operator A = I/dt - div(av(D) grad .) is symmetric, so each adjoint solve A mu = r is
+ * the same PT solve with homogeneous boundary values. D = H^3: the step Jacobian J is not symmetric; each
+ * adjoint solve J^T lam = r runs the transpose of the PT iteration linearised at that step's H (same
+ * damping, same convergence rate). Every adjoint solve stops at the reduction relative to its right-hand
+ * side that the forward solves reach (tol * dt / rms(H^nt) relative). Any output may be NULL except as
+ * stated. On success the context ends as after ptd_solve(nt); after an error (NaN, allocation) its fields
+ * are unspecified and should be set again. Allocates (nt + 4) grid-sized float64 buffers and a copy of the
+ * fluxes for the call (D = H^3: 3 + 2*ndim more), PTD_ERR_NOMEM if they do not fit. float64 only. */
 int32_t ptd_grad_solve(ptd_ctx* ctx, int32_t nt, const double* h_obs, int64_t count, double* loss,
                        double* grad_d, double* grad_h0, int64_t* iters_fwd, int64_t* iters_adj);
 

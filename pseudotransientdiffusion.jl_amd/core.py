@@ -158,20 +158,21 @@ class Context:
         return iters, errs
 
     def grad_solve(self, nt, h_obs):
-        """ptd_grad_solve: solve nt steps and return the loss L = 1/2 sum_inner (H - h_obs)^2, dL/dD and
-        dL/dH0 (reference layout) and the forward / adjoint PT iteration counts."""
+        """ptd_grad_solve: solve nt steps and return the loss L = 1/2 sum_inner (H - h_obs)^2, dL/dD (None for
+        D = H^3) and dL/dH0 (reference layout) and the forward / adjoint PT iteration counts."""
         shp = field_shape(self.params, _abi.F_H)
         hobs = np.asfortranarray(np.asarray(h_obs, dtype=np.float64))
         if hobs.shape != shp:
             raise ValueError(f"h_obs: expected shape {shp}, got {hobs.shape}")
         loss = ctypes.c_double()
-        gD = np.zeros(shp, order="F")
+        gD = None if self.params.dcase == _abi.NONLIN else np.zeros(shp, order="F")
         gH0 = np.zeros(shp, order="F")
         itf = np.zeros(nt, dtype=np.int64)
         ita = np.zeros(nt, dtype=np.int64)
         dp = ctypes.POINTER(ctypes.c_double)
         ip = ctypes.POINTER(ctypes.c_int64)
         check(self._lib.ptd_grad_solve(self.handle, int(nt), hobs.ctypes.data_as(dp), hobs.size, ctypes.byref(loss),
-                                       gD.ctypes.data_as(dp), gH0.ctypes.data_as(dp), itf.ctypes.data_as(ip),
+                                       None if gD is None else gD.ctypes.data_as(dp), gH0.ctypes.data_as(dp),
+                                       itf.ctypes.data_as(ip),
                                        ita.ctypes.data_as(ip)))
         return {"loss": loss.value, "grad_D": gD, "grad_H0": gH0, "iters_fwd": itf, "iters_adj": ita}

@@ -17,6 +17,7 @@
 // before it starts.
 #include <hip/hip_runtime.h>
 
+#include <algorithm>
 #include <cmath>
 #include <cstdint>
 #include <string>
@@ -56,7 +57,8 @@ extern "C" int32_t ptd_grad_solve(ptd_ctx* c, int32_t nt, const double* h_obs, i
     if (!c) return ptd::set_error(PTD_ERR_ARG, "ctx is NULL");
     if (nt < 1 || !h_obs) return ptd::set_error(PTD_ERR_ARG, "need nt >= 1 and H_obs");
     if (count != c->nH) return ptd::set_error(PTD_ERR_ARG, "H_obs must have nx*ny*nz values");
-    if (c->p.dcase == PTD_NONLIN) return ptd::set_error(PTD_ERR_STATE, "gradients are for D independent of H (lin / step)");
+    const bool nonlin = c->p.dcase == PTD_NONLIN;
+    if (nonlin && grad_d) return ptd::set_error(PTD_ERR_ARG, "dL/dD is not defined for D = H^3: pass grad_d = NULL");
     if (c->p.precision != PTD_FP64) return ptd::set_error(PTD_ERR_STATE, "gradients are float64");
     if (hipSetDevice(c->device) != hipSuccess) return ptd::set_error(PTD_ERR_HIP, "hipSetDevice failed");
 
@@ -68,7 +70,18 @@ extern "C" int32_t ptd_grad_solve(ptd_ctx* c, int32_t nt, const double* h_obs, i
     double* qbak[3] = {nullptr, nullptr, nullptr};
     double red[2] = {0, 0}, hsq[2] = {0, 0};
     const double tol_fwd = c->p.tol;
+    double *v = nullptr, *lam[2] = {nullptr, nullptr}, *u[3] = {nullptr, nullptr, nullptr};
+    double* z[3] = {nullptr, nullptr, nullptr};
     {
+        if (nonlin) {  // the transposed iteration's cell and face state, lam of two consecutive steps
+            PTD_ALLOCG(&v, nH * 8);
+            for (int b = 0; b < 2; ++b) PTD_ALLOCG(&lam[b], nH * 8);
+            for (int a = 0; a < c->p.ndim; ++a) {
+                PTD_ALLOCG(&u[a], nH * 8);
+                PTD_ALLOCG(&z[a], nH * 8);
+            }
+            for (int a = c->p.ndim; a < 3; ++a) u[a] = z[a] = u[0];  // absent axes: never read
+        }
         PTD_ALLOCG(&ckpt, (size_t)(nt + 1) * nH * 8);
         PTD_ALLOCG(&hobs, nH * 8);
         PTD_ALLOCG(&gD, nH * 8);
@@ -111,8 +124,51 @@ extern "C" int32_t ptd_grad_solve(ptd_ctx* c, int32_t nt, const double* h_obs, i
         // tol 0 and an adjoint of 0, whose first residual is already 0
         c->p.tol = hsq[0] > 0.0 ? tol_fwd * std::sqrt(2.0 * red[0]) * c->p.dt / std::sqrt(hsq[0]) : tol_fwd;
 
+        if (nonlin) {
+            // J_n^T lam_n = rhs_n/dt with rhs_nt = (H^nt - H_obs)*dt (in Hold), rhs_n = lam_{n+1}; each solve
+            // stops at the forward's relative reduction: rms(residual) <= tol*dt/rms(H^nt) * rms(rhs_n/dt).
+            // v and u carry over from one step to the next (warm start).
+            const double nR = (double)c->nR;
+            const double tol_rel = hsq[0] > 0.0 ? tol_fwd * c->p.dt / std::sqrt(hsq[0] / nR) : 0.0;
+            PTD_HIPG(hipMemsetAsync(v, 0, nH * 8, c->stream));
+            for (int a = 0; a < c->p.ndim; ++a) PTD_HIPG(hipMemsetAsync(u[a], 0, nH * 8, c->stream));
+            const double* rhs = (const double*)c->Hold;
+            double rhs_rms = std::sqrt(2.0 * red[0] / nR);  // rms(rhs/dt) = rms(H^nt - H_obs)
+            for (int n = nt; n >= 1; --n) {
+                const double* Hn = ckpt + (size_t)n * nH;
+                double* lamn = lam[n & 1];
+                if (n < nt) {  // rms(rhs_n/dt) of rhs_n = lam_{n+1}
+                    PTD_HIPG(ptd::launch_sumsq_inner(c, rhs));
+                    PTD_HIPG(hipMemcpyAsync(hsq, c->dred, 2 * sizeof(double), hipMemcpyDeviceToHost, c->stream));
+                    PTD_HIPG(hipStreamSynchronize(c->stream));
+                    rhs_rms = std::sqrt(hsq[0] / nR) / c->p.dt;
+                }
+                const double tol_n = tol_rel * rhs_rms;
+                int64_t it = 0;
+                double err = HUGE_VAL;
+                while (it < c->p.itmax) {
+                    const int64_t blk = std::min<int64_t>(c->p.nout, c->p.itmax - it);
+                    for (int64_t b = 0; b < blk; ++b) PTD_HIPG(ptd::launch_adjnl_iteration(c, Hn, v, u, z, rhs));
+                    it += blk;
+                    PTD_HIPG(ptd::launch_adjnl_lambda(c, Hn, v, lamn, rhs, true));
+                    PTD_HIPG(hipMemcpyAsync(red, c->dred, 2 * sizeof(double), hipMemcpyDeviceToHost, c->stream));
+                    PTD_HIPG(hipStreamSynchronize(c->stream));
+                    err = std::sqrt(red[0] / nR);
+                    if (std::isnan(err) || err <= tol_n) break;
+                }
+                if (iters_adj) iters_adj[n - 1] = it;
+                if (std::isnan(err)) {
+                    status = ptd::set_error(PTD_ERR_NAN, "NaN in the adjoint solve");
+                    goto done;
+                }
+                if (it == 0) PTD_HIPG(ptd::launch_adjnl_lambda(c, Hn, v, lamn, rhs, false));
+                PTD_HIPG(ptd::launch_adjnl_grad(c, Hn, lamn, gH0, n == 1));
+                rhs = lamn;
+            }
+        }
+
         // adjoint: mu_nt, ..., mu_1, each followed by its gradient contribution
-        for (int n = nt; n >= 1; --n) {
+        for (int n = nt; n >= 1 && !nonlin; --n) {
             int64_t it = 0;
             double rms = 0, mx = 0;
             status = ptd_pt_solve(c, &it, &rms, &mx);
@@ -139,6 +195,13 @@ extern "C" int32_t ptd_grad_solve(ptd_ctx* c, int32_t nt, const double* h_obs, i
 done:
     c->p.tol = tol_fwd;
     (void)hipStreamSynchronize(c->stream);
+    (void)hipFree(v);
+    for (int b = 0; b < 2; ++b) (void)hipFree(lam[b]);
+    for (int a = 0; a < 3; ++a)
+        if (a < c->p.ndim) {
+            (void)hipFree(u[a]);
+            (void)hipFree(z[a]);
+        }
     (void)hipFree(ckpt);
     (void)hipFree(hobs);
     (void)hipFree(gD);

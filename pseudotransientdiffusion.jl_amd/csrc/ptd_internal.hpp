@@ -507,6 +507,14 @@ hipError_t launch_adj_source(ptd_ctx* c, const double* H, const double* Hobs, do
 hipError_t launch_grad_accum(ptd_ctx* c, const double* mu, const double* Hn, double* gD, double* gH0, bool first);
 // sum of H^2 over the inner cells into c->dred
 hipError_t launch_sumsq_inner(ptd_ctx* c, const double* H);
+// adjoint of the nonlinear steps at the converged H of one step (ptd_grad.hip): one transposed PT
+// iteration on (u faces, v cells) with right-hand side rhs/dt; lam = rho v (and with `resid` the residual
+// rhs/dt - J^T lam reduced into c->dred); the step's contribution to dL/dH0
+hipError_t launch_adjnl_iteration(ptd_ctx* c, const double* H, double* v, double* const u[3], double* const z[3],
+                                  const double* rhs);
+hipError_t launch_adjnl_lambda(ptd_ctx* c, const double* H, const double* v, double* lam, const double* rhs,
+                               bool resid);
+hipError_t launch_adjnl_grad(ptd_ctx* c, const double* H, const double* lam, double* gH0, bool first);
 
 // ---- batches of independent float64 solves (ptd_batch.hip) -----------------------------------
 

@@ -1876,6 +1876,228 @@ k_grad_accum(KP<double> p, const double* __restrict__ mu, const double* __restri
     }
 }
 
+// ---- adjoint of the nonlinear (D = H^3) steps ---------------------------------------------------
+//
+// One backward-Euler step solves F(H) = (H - Hold)/dt + div q(H) = 0 on the inner cells with
+// q_f = -G_f, G_f = c_f (H_b - H_a)/h, c_f = av^3, av = (H_a + H_b)*0.5 (scripts/diff_3D/
+// diff_3D_nonlin_multixpu.jl:37-50). Its Jacobian J = I/dt - S J_G/h (S: cell-face incidence, J_G = dG/dH)
+// is not symmetric: dG_f/dH_a = (1.5 av^2 dH - c_f)/h, dG_f/dH_b = (1.5 av^2 dH + c_f)/h. The adjoint
+// J^T lam = rhs is solved by the transpose of the forward PT iteration linearised at the converged H
+// (theta_f and dtau frozen there; the same spectrum, so the same convergence rate):
+//   face f:  w = u - (rho_a v_a - rho_b v_b)/h,  z = w/(1 + theta),  u <- theta z
+//   cell j:  v <- v/(1 + dtau/dt) - sum_f dG_f/dH_j z_f + rhs_j/dt        (inner cells; v = 0 elsewhere)
+// with rho = dtau/(1 + dtau/dt); its fixed point gives lam = rho v. Face values are stored in grid-sized
+// arrays at the index of the face's upper cell b (a = b - stride); a face exists where the reference's
+// flux array has one (transverse indices inner). theta and dtau are the plain-form expressions
+// (face_theta, nonlin_dtau).
+
+__device__ __forceinline__ double adj_rho(const KP<double>& p, double H) {
+    const double dtau = nonlin_dtau<double, PTD_PLAIN>(p, H);
+    return dtau / (1.0 + dtau / p.dt);
+}
+
+// (dG/dH_a, dG/dH_b) and theta of the face between cells with values Ha (lower) and Hb (upper)
+__device__ __forceinline__ void adj_face_coefs(const KP<double>& p, double Ha, double Hb, double ih, double& ga,
+                                               double& gb, double& theta) {
+    const double av = (Ha + Hb) * 0.5;
+    const double c = av * av * av;
+    const double k = 1.5 * av * av * (Hb - Ha);
+    ga = (k - c) * ih;
+    gb = (k + c) * ih;
+    theta = face_theta<double, PTD_NONLIN, PTD_PLAIN>(p, c);
+}
+
+struct AdjGeo {
+    int64_t i, j, k, st[3], n[3];
+    int ndim;
+    __device__ AdjGeo(const KP<double>& p, int64_t c, int nd) : ndim(nd) {
+        split3(c, p.nx, p.ny, i, j, k);
+        st[0] = 1;
+        st[1] = p.nx;
+        st[2] = p.nx * p.ny;
+        n[0] = p.nx;
+        n[1] = p.ny;
+        n[2] = p.nz;
+    }
+    __device__ int64_t idx(int ax) const { return ax == 0 ? i : ax == 1 ? j : k; }
+    __device__ bool inner_ax(int ax) const { return idx(ax) >= 1 && idx(ax) <= n[ax] - 2; }
+    __device__ bool inner() const {
+        bool in = inner_ax(0);
+        if (ndim >= 2) in = in && inner_ax(1);
+        if (ndim >= 3) in = in && inner_ax(2);
+        return in;
+    }
+    // the face (c - st | c) along ax exists: transverse indices inner and c not on the lower edge
+    __device__ bool lower_face(int ax) const {
+        if (idx(ax) < 1) return false;
+        for (int o = 0; o < ndim; ++o)
+            if (o != ax && !inner_ax(o)) return false;
+        return true;
+    }
+};
+
+template <int NDIM>
+__global__ void __launch_bounds__(kBlock)
+k_adjnl_face(KP<double> p, const double* __restrict__ H, const double* __restrict__ v, double* __restrict__ u0,
+             double* __restrict__ u1, double* __restrict__ u2, double* __restrict__ z0, double* __restrict__ z1,
+             double* __restrict__ z2, int64_t n) {
+    const int64_t c = (int64_t)blockIdx.x * kBlock + threadIdx.x;
+    if (c >= n) return;
+    const AdjGeo g(p, c, NDIM);
+    double* const u[3] = {u0, u1, u2};
+    double* const z[3] = {z0, z1, z2};
+    const double ih[3] = {p._dx, p._dy, p._dz};
+    const bool cin = g.inner();
+    const double rb = cin ? adj_rho(p, H[c]) * v[c] : 0.0;
+#pragma unroll
+    for (int ax = 0; ax < NDIM; ++ax) {
+        if (!g.lower_face(ax)) continue;
+        const int64_t a = c - g.st[ax];
+        double ga, gb, th;
+        adj_face_coefs(p, H[a], H[c], ih[ax], ga, gb, th);
+        bool ain = true;
+        for (int o = 0; o < NDIM; ++o) ain = ain && (o == ax ? g.idx(o) - 1 >= 1 : g.inner_ax(o));
+        const double ra = ain ? adj_rho(p, H[a]) * v[a] : 0.0;
+        const double w = u[ax][c] - (ra - rb) * ih[ax];
+        const double zz = w / (1.0 + th);
+        z[ax][c] = zz;
+        u[ax][c] = th * zz;
+    }
+}
+
+template <int NDIM>
+__global__ void __launch_bounds__(kBlock)
+k_adjnl_cell(KP<double> p, const double* __restrict__ H, double* __restrict__ v, const double* __restrict__ z0,
+             const double* __restrict__ z1, const double* __restrict__ z2, const double* __restrict__ rhs, int64_t n) {
+    const int64_t c = (int64_t)blockIdx.x * kBlock + threadIdx.x;
+    if (c >= n) return;
+    const AdjGeo g(p, c, NDIM);
+    if (!g.inner()) return;
+    const double* const z[3] = {z0, z1, z2};
+    const double ih[3] = {p._dx, p._dy, p._dz};
+    const double Hc = H[c];
+    double acc = 0.0;
+#pragma unroll
+    for (int ax = 0; ax < NDIM; ++ax) {
+        const int64_t lo = c - g.st[ax], up = c + g.st[ax];
+        double ga, gb, th;
+        adj_face_coefs(p, H[lo], Hc, ih[ax], ga, gb, th);  // lower face: c is its upper cell b
+        acc += gb * z[ax][c];
+        adj_face_coefs(p, Hc, H[up], ih[ax], ga, gb, th);  // upper face: c is its lower cell a
+        acc += ga * z[ax][up];
+    }
+    const double dtau = nonlin_dtau<double, PTD_PLAIN>(p, Hc);
+    v[c] = v[c] / (1.0 + dtau / p.dt) - acc + rhs[c] * p._dt;
+}
+
+// lam = rho v on the inner cells (0 on the boundary); with `resid` also the residual
+// rhs/dt - J^T lam, per-block (sum r^2, max |r|).
+template <int NDIM>
+__global__ void __launch_bounds__(kBlock)
+k_adjnl_lambda(KP<double> p, const double* __restrict__ H, const double* __restrict__ v, double* __restrict__ lam,
+               const double* __restrict__ rhs, double2* __restrict__ partial, int resid, int64_t n) {
+    const int64_t c = (int64_t)blockIdx.x * kBlock + threadIdx.x;
+    double s = 0.0, m = 0.0;
+    if (c < n) {
+        const AdjGeo g(p, c, NDIM);
+        if (g.inner()) {
+            const double lc = adj_rho(p, H[c]) * v[c];
+            lam[c] = lc;
+            if (resid) {
+                const double ih[3] = {p._dx, p._dy, p._dz};
+                double jt = lc * p._dt;
+#pragma unroll
+                for (int ax = 0; ax < NDIM; ++ax) {
+                    const int64_t lo = c - g.st[ax], up = c + g.st[ax];
+                    const AdjGeo gl(p, lo, NDIM), gu(p, up, NDIM);
+                    const double ll = gl.inner() ? adj_rho(p, H[lo]) * v[lo] : 0.0;
+                    const double lu = gu.inner() ? adj_rho(p, H[up]) * v[up] : 0.0;
+                    double ga, gb, th;
+                    adj_face_coefs(p, H[lo], H[c], ih[ax], ga, gb, th);
+                    jt += gb * (lc - ll) * ih[ax];  // face (lo | c): dG/dH_b (lam_b - lam_a)/h
+                    adj_face_coefs(p, H[c], H[up], ih[ax], ga, gb, th);
+                    jt += ga * (lu - lc) * ih[ax];  // face (c | up): dG/dH_a (lam_b - lam_a)/h
+                }
+                const double r = rhs[c] * p._dt - jt;
+                s = r * r;
+                m = fabs(r);
+            }
+        } else {
+            lam[c] = 0.0;
+        }
+    }
+    if (resid) block_sum_max(s, m, threadIdx.x, kBlock, &partial[blockIdx.x]);
+}
+
+// Boundary cells: gH0 += sum over their faces of dG_f/dH_c (lam_a - lam_b)/h (lam = 0 on the boundary); inner
+// cells on the first step: gH0 += lam/dt.
+template <int NDIM>
+__global__ void __launch_bounds__(kBlock)
+k_adjnl_grad(KP<double> p, const double* __restrict__ H, const double* __restrict__ lam, double* __restrict__ gH0,
+             int first, int64_t n) {
+    const int64_t c = (int64_t)blockIdx.x * kBlock + threadIdx.x;
+    if (c >= n) return;
+    const AdjGeo g(p, c, NDIM);
+    if (g.inner()) {
+        if (first) gH0[c] += lam[c] * p._dt;
+        return;
+    }
+    const double ih[3] = {p._dx, p._dy, p._dz};
+    double acc = 0.0;
+#pragma unroll
+    for (int ax = 0; ax < NDIM; ++ax) {
+        bool trans = true;
+        for (int o = 0; o < NDIM; ++o)
+            if (o != ax) trans = trans && g.inner_ax(o);
+        if (!trans) continue;
+        double ga, gb, th;
+        if (g.idx(ax) >= 1) {  // face (c - st | c): c is b
+            const int64_t a = c - g.st[ax];
+            adj_face_coefs(p, H[a], H[c], ih[ax], ga, gb, th);
+            acc += gb * lam[a] * ih[ax];
+        }
+        if (g.idx(ax) <= g.n[ax] - 2) {  // face (c | c + st): c is a
+            const int64_t b = c + g.st[ax];
+            adj_face_coefs(p, H[c], H[b], ih[ax], ga, gb, th);
+            acc -= ga * lam[b] * ih[ax];
+        }
+    }
+    gH0[c] += acc;
+}
+
+#define PTD_ADJ_ND(KERNEL, ...)                                                                           \
+    do {                                                                                                  \
+        const KP<double> kp = make_kp<double>(c->p);                                                      \
+        const dim3 g(grid_for(c->nH)), bl(kBlock);                                                        \
+        if (c->p.ndim == 1) hipLaunchKernelGGL((KERNEL<1>), g, bl, 0, c->stream, kp, __VA_ARGS__, c->nH); \
+        else if (c->p.ndim == 2) hipLaunchKernelGGL((KERNEL<2>), g, bl, 0, c->stream, kp, __VA_ARGS__, c->nH); \
+        else hipLaunchKernelGGL((KERNEL<3>), g, bl, 0, c->stream, kp, __VA_ARGS__, c->nH);                \
+    } while (0)
+
+hipError_t launch_adjnl_iteration(ptd_ctx* c, const double* H, double* v, double* const u[3], double* const z[3],
+                                  const double* rhs) {
+    PTD_ADJ_ND(k_adjnl_face, H, (const double*)v, u[0], u[1], u[2], z[0], z[1], z[2]);
+    PTD_ADJ_ND(k_adjnl_cell, H, v, (const double*)z[0], (const double*)z[1], (const double*)z[2], rhs);
+    return hipGetLastError();
+}
+
+hipError_t launch_adjnl_lambda(ptd_ctx* c, const double* H, const double* v, double* lam, const double* rhs,
+                               bool resid) {
+    const unsigned g = grid_for(c->nH);
+    if ((int64_t)g > c->npartial) return hipErrorInvalidValue;
+    PTD_ADJ_ND(k_adjnl_lambda, H, v, lam, rhs, (double2*)c->partial, (int)resid);
+    if (resid)
+        hipLaunchKernelGGL(k_reduce_partials, dim3(1), dim3(kBlock), 0, c->stream, (const double2*)c->partial,
+                           (int64_t)g, c->dred);
+    return hipGetLastError();
+}
+
+hipError_t launch_adjnl_grad(ptd_ctx* c, const double* H, const double* lam, double* gH0, bool first) {
+    PTD_ADJ_ND(k_adjnl_grad, H, lam, gH0, (int)first);
+    return hipGetLastError();
+}
+#undef PTD_ADJ_ND
+
 hipError_t launch_adj_source(ptd_ctx* c, const double* H, const double* Hobs, double* out) {
     const KP<double> kp = make_kp<double>(c->p);
     const unsigned g = grid_for(c->nH);
@@ -1889,6 +2111,7 @@ hipError_t launch_adj_source(ptd_ctx* c, const double* H, const double* Hobs, do
 hipError_t launch_sumsq_inner(ptd_ctx* c, const double* H) {
     const KP<double> kp = make_kp<double>(c->p);
     const unsigned g = grid_for(c->nH);
+    if ((int64_t)g > c->npartial) return hipErrorInvalidValue;
     hipLaunchKernelGGL(k_sumsq_inner, dim3(g), dim3(kBlock), 0, c->stream, kp, H, (double2*)c->partial, c->nH);
     hipLaunchKernelGGL(k_reduce_partials, dim3(1), dim3(kBlock), 0, c->stream, (const double2*)c->partial, (int64_t)g,
                        c->dred);

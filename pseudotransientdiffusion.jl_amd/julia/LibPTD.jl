@@ -176,16 +176,18 @@ end
 """
     grad_solve!(ctx, nt, Hobs) -> (L, dL/dD, dL/dH0, iters_fwd, iters_adj)
 
-nt physical steps of a lin / step context and the gradient of L = 1/2 sum_inner (H(T) - H_obs)^2 by the
-adjoint (no reference counterpart). The context ends as after `solve!(ctx, nt)`.
+nt physical steps of the context's problem and the gradient of L = 1/2 sum_inner (H(T) - H_obs)^2 by the
+adjoint (no reference counterpart); dL/dD is `nothing` for D = H^3. The context ends as after
+`solve!(ctx, nt)`.
 """
 function grad_solve!(ctx::Context, nt::Integer, Hobs::Array{Float64})
     L = Ref(0.0)
-    gD, gH0 = similar(Hobs), similar(Hobs)
+    nonlin = ctx.params.dcase == PTD_NONLIN
+    gD, gH0 = nonlin ? nothing : similar(Hobs), similar(Hobs)
     itf, ita = zeros(Int64, nt), zeros(Int64, nt)
     check(ccall((:ptd_grad_solve, libptd), Int32,
                 (Ptr{Cvoid}, Int32, Ptr{Float64}, Int64, Ref{Float64}, Ptr{Float64}, Ptr{Float64}, Ptr{Int64}, Ptr{Int64}),
-                ctx.handle, nt, Hobs, length(Hobs), L, gD, gH0, itf, ita))
+                ctx.handle, nt, Hobs, length(Hobs), L, nonlin ? Ptr{Float64}(C_NULL) : pointer(gD), gH0, itf, ita))
     return L[], gD, gH0, itf, ita
 end
 

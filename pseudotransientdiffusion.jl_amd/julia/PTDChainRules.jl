@@ -1,4 +1,4 @@
-# PTDChainRules.jl -- a ChainRulesCore rrule for the loss of an nt-step lin / step solve on libptd
+# PTDChainRules.jl -- a ChainRulesCore rrule for the loss of an nt-step solve on libptd (lin, step, D = H^3)
 # (no reference counterpart: the reference has no derivative). The pullback scales the adjoint gradients
 # ptd_grad_solve returns (dL/dD, dL/dH0) by the incoming cotangent.
 #
@@ -21,7 +21,7 @@ end
 """
     solve_loss(ctx, D, H0, Hobs, nt) = 1/2 sum_inner (H(T) - Hobs)^2
 
-H(T) after nt backward-Euler steps of the context's lin / step problem from H0 (with D the step field).
+H(T) after nt backward-Euler steps of the context's problem from H0 (with D the step field).
 """
 function solve_loss(ctx, D, H0, Hobs, nt)
     prepare!(ctx, D, H0)
@@ -33,7 +33,8 @@ end
 function ChainRulesCore.rrule(::typeof(solve_loss), ctx, D, H0, Hobs, nt)
     prepare!(ctx, D, H0)
     L, gD, gH0, _, _ = LibPTD.grad_solve!(ctx, nt, Hobs)
-    pullback(ΔL) = (NoTangent(), NoTangent(), ΔL .* gD, ΔL .* gH0, @not_implemented("dL/dHobs"), NoTangent())
+    dD(ΔL) = gD === nothing ? NoTangent() : ΔL .* gD   # D = H^3: no D parameter
+    pullback(ΔL) = (NoTangent(), NoTangent(), dD(ΔL), ΔL .* gH0, @not_implemented("dL/dHobs"), NoTangent())
     return L, pullback
 end
 

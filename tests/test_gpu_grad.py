@@ -143,3 +143,99 @@ def test_gradient_on_the_smallest_grids(script, shape, pkg):
     assert abs(g["loss"] - L) <= 1e-9 * max(L, 1e-300)
     assert np.max(np.abs(g["grad_D"] - gD)) <= 1e-6 * max(np.max(np.abs(gD)), 1e-300)
     assert np.max(np.abs(g["grad_H0"] - gH0)) <= 1e-6 * max(np.max(np.abs(gH0)), 1e-300)
+
+
+# ---- the nonlinear case (D = H^3): the transposed-Jacobian adjoint ----------------------------------
+
+from oracle.be_nonlin import BENonlin  # noqa: E402
+
+
+def _setup_nonlin(pkg, script, shape, tol, seed=0):
+    """H0 = 0.5 + exp(-r^2/4) (positive everywhere, so D = H^3 stays away from 0), H_obs a 5% perturbation."""
+    spec = pkg.SCRIPTS[script]
+    p = pkg.make_params(spec, *shape, arith=1)
+    p.tol = tol
+    nd = spec["ndim"]
+    ls = [spec["lx"], spec.get("ly", 0.0), spec.get("lz", 0.0)][:nd]
+    r2 = 0.0
+    for d in range(nd):
+        dx = ls[d] / shape[d]
+        xc = dx / 2 + dx * np.arange(shape[d]) - 0.5 * ls[d]
+        sh = [1] * nd
+        sh[d] = shape[d]
+        r2 = r2 + (xc ** 2).reshape(sh)
+    H0 = np.asfortranarray(0.5 + np.exp(-r2 / 4.0))
+    rng = np.random.default_rng(seed)
+    Hobs = np.asfortranarray(H0 * (1.0 + 0.05 * rng.standard_normal(shape)))
+    h = [p.dx, p.dy, p.dz][:nd]
+    return p, H0, Hobs, h
+
+
+@pytest.mark.parametrize("script,shape", [("diff_1D_nonlin", (64,)), ("diff_2D_nonlin", (40, 36)),
+                                          ("diff_3D_nonlin_multixpu", (24, 22, 20))])
+def test_nonlin_gradient_matches_oracle_adjoint_and_forward_mode(script, shape, pkg):
+    """dL/dH0 through the transposed-Jacobian adjoint against the exact discrete adjoint of Newton's
+    backward-Euler steps (oracle/be_nonlin.py) and 10 forward-mode directional derivatives."""
+    print(f"[gpu] nonlinear gradient vs oracle {script} {shape}", flush=True)
+    nt = 3
+    p, H0, Hobs, h = _setup_nonlin(pkg, script, shape, tol=1e-12)
+    g = _grad_gpu(pkg, p, H0, None, Hobs, nt)
+    assert g["grad_D"] is None
+    be = BENonlin(shape, h, p.dt)
+    L, gH0 = be.gradient(H0, Hobs, nt)
+    assert abs(g["loss"] - L) <= 1e-9 * L
+    assert np.max(np.abs(g["grad_H0"] - gH0)) <= 1e-6 * np.max(np.abs(gH0)), \
+        np.max(np.abs(g["grad_H0"] - gH0)) / np.max(np.abs(gH0))
+    rng = np.random.default_rng(7)
+    for _ in range(10):
+        v = rng.standard_normal(shape)
+        jv = be.jvp(H0, Hobs, nt, v)
+        gv = float(np.sum(g["grad_H0"] * v))
+        assert abs(gv - jv) <= 1e-6 * max(abs(jv), 1e-30), (gv, jv)
+
+
+def test_nonlin_gradient_matches_finite_differences_on_gpu(pkg):
+    print("[gpu] nonlinear gradient vs finite differences 48^3", flush=True)
+    script, shape, nt = "diff_3D_nonlin_multixpu", (48, 48, 48), 3
+    p, H0, Hobs, h = _setup_nonlin(pkg, script, shape, tol=1e-12, seed=1)
+    g = _grad_gpu(pkg, p, H0, None, Hobs, nt)
+    inner = (slice(1, -1),) * 3
+
+    def loss(H0v):
+        with pkg.Context(p) as ctx:
+            ctx.set("H", H0v)
+            ctx.set("Hold", H0v)
+            ctx.solve(nt)
+            r = ctx.get("H")[inner] - Hobs[inner]
+            return 0.5 * float(np.sum(r * r))
+
+    rng = np.random.default_rng(11)
+    for _ in range(4):
+        v = np.asfortranarray(rng.standard_normal(shape) * 0.1)
+        eps = 1e-5
+        fd = (loss(H0 + eps * v) - loss(H0 - eps * v)) / (2 * eps)
+        gv = float(np.sum(g["grad_H0"] * v))
+        assert abs(fd - gv) <= 1e-5 * abs(gv), (fd, gv)
+
+
+def test_nonlin_gradient_context_and_arguments(pkg):
+    """The context ends as after ptd_solve(nt); dL/dD is refused for D = H^3 (PTD_ERR_ARG)."""
+    import ctypes
+    script, shape, nt = "diff_3D_nonlin_multixpu", (30, 26, 22), 3
+    p, H0, Hobs, h = _setup_nonlin(pkg, script, shape, tol=1e-8)
+    states = []
+    for use_grad in (True, False):
+        with pkg.Context(p) as ctx:
+            ctx.set("H", H0)
+            ctx.set("Hold", H0)
+            its = ctx.grad_solve(nt, Hobs)["iters_fwd"] if use_grad else ctx.solve(nt)[0]
+            states.append((its, ctx.get("H"), ctx.get("Hold"), ctx.get("qHx"), ctx.get("qHz"), ctx.solve(1)[0]))
+    for a, b in zip(*states):
+        assert np.array_equal(a, b)
+    with pkg.Context(p) as ctx:
+        lib = pkg.load()
+        gD = np.zeros(shape, order="F")
+        dp = ctypes.POINTER(ctypes.c_double)
+        st = lib.ptd_grad_solve(ctx.handle, nt, Hobs.ctypes.data_as(dp), Hobs.size, None, gD.ctypes.data_as(dp),
+                                None, None, None)
+        assert st == 1 and b"grad_d = NULL" in lib.ptd_last_error()
