@@ -801,12 +801,18 @@ __device__ __forceinline__ void fused3d_t2_tile(const KP<T>& p, const T* __restr
     const size_t psQx = (size_t)(nx - 1) * (ny - 2);
     const size_t psQy = (size_t)(nx - 2) * (ny - 1);
     const size_t psQz = (size_t)(nx - 2) * (ny - 2);
-    const int ic = max(0, min(i, nx - 1)), jc = max(0, min(j, ny - 1));
-    const int ixq = max(1, min(i, nx - 1)), jxq = max(1, min(j, ny - 2));
-    const int iyq = max(1, min(i, nx - 2)), jyq = max(1, min(j, ny - 1));
-    const int jzq = max(1, min(j, ny - 2));
+    // Hold, qx and qz are used only on the rows that update a level-1 cell (1..R-2) and qy only on the rows
+    // that evaluate their lower y face (1..R-1): the other rows load them from the nearest such row, so the
+    // tile's halo rows fetch no bytes beyond H (the same instructions, cache hits instead of new lines).
+    const int jr = (R - 4) * by - 1 + max(1, min(row, R - 2));
+    const int jq = (R - 4) * by - 1 + max(1, row);
+    const int ic = max(0, min(i, nx - 1)), jc = max(0, min(j, ny - 1)), jo = max(0, min(jr, ny - 1));
+    const int ixq = max(1, min(i, nx - 1)), jxq = max(1, min(jr, ny - 2));
+    const int iyq = max(1, min(i, nx - 2)), jyq = max(1, min(jq, ny - 1));
+    const int jzq = max(1, min(jr, ny - 2));
     constexpr uint32_t es = (uint32_t)sizeof(T);
     const uint32_t oH = (uint32_t)(ic + nx * jc) * es;
+    const uint32_t oHo = (uint32_t)(ic + nx * jo) * es;
     const uint32_t oA = (uint32_t)(ic + nx * jc) * (uint32_t)sizeof(Ta);
     const uint32_t oQx = (uint32_t)((ixq - 1) + (nx - 1) * (jxq - 1)) * es;
     const uint32_t oQy = (uint32_t)((iyq - 1) + (nx - 2) * (jyq - 1)) * es;
@@ -841,7 +847,7 @@ __device__ __forceinline__ void fused3d_t2_tile(const KP<T>& p, const T* __restr
         // R-1 skip these loads). Measured: -1% on the float32 correction sweep, but +4% on the float64
         // nonlinear and +18% on the float64 step sweeps, whose loads then lose their scheduling.
         constexpr bool kSkip = sizeof(T) == 4;
-        o.Ho = (!kSkip || l1all) ? ldc<T>(bHo, oH + (uint32_t)(pl - kb) * pbH) : T{};
+        o.Ho = (!kSkip || l1all) ? ldc<T>(bHo, oHo + (uint32_t)(pl - kb) * pbH) : T{};
         o.qx = (!kSkip || l1all) ? ldc<T>(bqx, oQx + pq * pbQx) : T{};
         o.qy = (!kSkip || l1y) ? ldc<T>(bqy, oQy + pq * pbQy) : T{};
         o.qz = (!kSkip || l1all) ? ldc<T>(bqz, oQz + pz * pbQz) : T{};
