@@ -130,13 +130,15 @@ def batch_leg(pkg, count, seed=0):
         log(f"batch: solving ({b.device_bytes / 1e9:.1f} GB on the device)")
         it, er, lat, st, total_ms = b.solve(5)
         tz, rows, cand_ms = b.tiling()
+        groups = b.group_times()
     gb = sum(B.member_a_eff_gb(m) * int(it[i].sum()) for i, m in enumerate(w))
     cells = sum(m.n ** (2 if "2D" in m.script else 3) for m in w)
     out = {"members": count, "seed": seed, "cells": int(cells), "physical_steps": 5,
            "wall_s": round(total_ms / 1e3, 4), "T_eff_aggregate_GBs": round(gb / (total_ms / 1e3), 2),
            "latency_ms_p50": round(B.percentile(lat, 50), 2), "latency_ms_p99": round(B.percentile(lat, 99), 2),
            "pt_iterations_total": int(it.sum()), "all_converged": bool((st == 0).all() and (er <= 1e-8).all()),
-           "tiling": {"tz3d": tz, "rows2d": rows, "candidate_block_ms": [round(v, 3) for v in cand_ms]}}
+           "tiling": {"tz3d": tz, "rows2d": rows, "candidate_block_ms_median": [round(v, 3) for v in cand_ms]},
+           "group_ms": groups}
     log(f"batch: {out['wall_s']} s, aggregate T_eff {out['T_eff_aggregate_GBs']} GB/s, "
         f"latency p50 {out['latency_ms_p50']} ms p99 {out['latency_ms_p99']} ms")
     return out

@@ -230,10 +230,10 @@ int32_t ptd_grad_solve(ptd_ctx* ctx, int32_t nt, const double* h_obs, int64_t co
  * No reference counterpart: the reference solves one problem per script run. A batch holds n
  * independent float64 problems (2D or 3D, lin / step / nonlin, each with its own grid and scalars,
  * the converging scripts' PTD_PLAIN form, one nout for all, itmax a multiple of nout, nout even) and
- * solves them together: one launch per (ndim, dcase, arith) group covers every member's tiles, each
- * member's PT loop (checks, stopping, Hold .= H, physical steps) runs on the device, and one check
- * block is a replayed HIP graph. Every member's result, iteration counts included, is bit-identical to
- * ptd_solve on that problem alone. */
+ * solves them together: one launch per (sweep kind, dcase, arith) group covers every member's tiles, each
+ * member's PT loop (checks, stopping, Hold .= H, physical steps) runs on the device, and each group's check
+ * block is a HIP graph replayed on the group's own stream (the groups advance independently). Every
+ * member's result, iteration counts included, is bit-identical to ptd_solve on that problem alone. */
 typedef struct ptd_batch ptd_batch;
 
 /* params[n], 1 <= n <= 1024. Allocates every member's fields (zeroed) on `device`. */
@@ -249,12 +249,19 @@ int32_t ptd_batch_get_field(ptd_batch* batch, int32_t member, int32_t field, dou
  * per-member, per-step iteration counts and final errors (member-major), latency_ms[n] the time from
  * the start of the call until each member finished, status[n] PTD_OK or PTD_ERR_NAN, *total_ms the
  * wall time of the call; any output may be NULL. The first call also picks the tiling of the
- * non-checking launches by timing one check block with each candidate (results do not depend on it). */
+ * non-checking launches by timing check blocks with each candidate in turn, three rounds, keeping the least
+ * median (results do not depend on it). */
 int32_t ptd_batch_solve(ptd_batch* batch, int32_t nt, int64_t* iters, double* errs, double* latency_ms,
                         int32_t* status, double* total_ms);
 /* The tiling chosen by the first solve: z chunk depth of the 3D launches and rows per block of the 2D
  * launches, and the measured time of one check block per candidate (PTD_ERR_STATE before a solve). */
 int32_t ptd_batch_tiling(const ptd_batch* batch, int32_t* tz3d, int32_t* rows2d, double* candidate_ms, int32_t ncand);
+/* Where the last ptd_batch_solve spent its time, per member group (members sharing a sweep kind, D case and
+ * arithmetic run as one launch per sweep): *ngroups, and for the first `cap` groups the sweep kind (3: 3D
+ * two-level, 22: 2D two-level, 2: 2D single-level), the D case, the member count and the device time the
+ * group's stream took, summed over the replays (ms; the groups run concurrently, so the times overlap). */
+int32_t ptd_batch_group_times(const ptd_batch* batch, int32_t* ngroups, int32_t* kind, int32_t* dcase,
+                              int32_t* members, double* ms, int32_t cap);
 
 #ifdef __cplusplus
 }

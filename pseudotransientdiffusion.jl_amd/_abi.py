@@ -35,7 +35,7 @@ EXPORTS = (
     "ptd_iterate", "ptd_iterate_timed", "ptd_pt_solve", "ptd_solve",
     "ptd_grad_solve", "ptd_pt_solve_noalloc",
     "ptd_batch_create", "ptd_batch_destroy", "ptd_batch_size", "ptd_batch_device_bytes",
-    "ptd_batch_set_field", "ptd_batch_get_field", "ptd_batch_solve", "ptd_batch_tiling",
+    "ptd_batch_set_field", "ptd_batch_get_field", "ptd_batch_solve", "ptd_batch_tiling", "ptd_batch_group_times",
 )
 
 
@@ -110,6 +110,8 @@ def load():
         "ptd_batch_get_field": (i32, [vp, i32, i32, dp, i64]),
         "ptd_batch_solve": (i32, [vp, i32, i64p, dp, dp, ctypes.POINTER(i32), dp]),
         "ptd_batch_tiling": (i32, [vp, ctypes.POINTER(i32), ctypes.POINTER(i32), dp, i32]),
+        "ptd_batch_group_times": (i32, [vp, ctypes.POINTER(i32), ctypes.POINTER(i32), ctypes.POINTER(i32),
+                                        ctypes.POINTER(i32), dp, i32]),
     }
     for name, (res, args) in sig.items():
         fn = getattr(L, name)

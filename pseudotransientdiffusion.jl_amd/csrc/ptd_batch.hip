@@ -7,16 +7,19 @@
 //  - members are grouped by (sweep kind, dcase, arith), the kind being the sweep ptd_solve runs for
 //    them (two-level 3D, two-level 2D on large grids, single-level 2D); one launch covers the tiles of
 //    every member of a group (a table maps each block to its member, tile and z chunk depth);
-//  - the PT loop of every member runs on the device: after each check block of nout iterations a
-//    controller kernel applies the reference loop's rules per member (scripts/diff_3D/
+//  - the PT loop of every member runs on the device: after each check block of nout iterations the
+//    group's controller kernel applies the reference loop's rules per member (scripts/diff_3D/
 //    diff_3D_nonlin_multixpu.jl:104-118) and members that are done drop out of later launches;
-//  - one check block (all groups on their own streams, then reduction, controller and Hold .= H) is
-//    captured once as a HIP graph and replayed until no member is running; groups of 2D grids wider
-//    than the 3D ones run several check blocks per replay (group_sub_blocks), as they need
-//    proportionally more PT iterations;
-//  - the tiling of the non-checking launches (z chunk depth in 3D, rows per block in 2D) is chosen
-//    online: the first replays try each candidate graph (each replay is real progress, results do not
-//    depend on the tiling) and the fastest is kept.
+//  - each group's check block (its launches, reduction, controller, Hold .= H and its running-member
+//    count) is captured once as a HIP graph on the group's own stream and replayed there; the groups
+//    advance independently (no group waits for a slower one), the host polling their counts every few
+//    replays; groups of 2D grids wider than the 3D ones run several check blocks per replay
+//    (group_sub_blocks), as they need proportionally more PT iterations;
+//  - the tiling of the non-checking launches (z chunk depth in 3D, rows per block in 2D) is chosen online
+//    per group: the first replays try each candidate graph in turn, three rounds (each replay is real
+//    progress, results do not depend on the tiling), and the candidate of least median replay time is kept;
+//  - each group's graph brackets its launches with two clock kernels that sum the device time the group
+//    took over the replays (ptd_batch_group_times).
 #include <hip/hip_runtime.h>
 
 #include <algorithm>
@@ -24,6 +27,7 @@
 #include <cmath>
 #include <cstring>
 #include <string>
+#include <thread>
 #include <vector>
 
 #include "ptd_internal.hpp"
@@ -52,6 +56,16 @@ struct Group {
     BatchTile* d_t2 = nullptr;  // single-level 2D groups: two-level tiles of their pair launches
     int n_t2 = 0;
     int sub = 1;                // check blocks per replay
+    uint64_t* d_span = nullptr; // (start, sum) of the group's device time per replay, summed over replays
+    int32_t* d_active = nullptr; // members of the group still running, after each replay
+    int32_t* h_active = nullptr; // its pinned host mirror (a slot of ptd_batch::h_active)
+    hipEvent_t ev_a = nullptr, ev_b = nullptr;
+    hipEvent_t ev_poll[2] = {nullptr, nullptr};  // ends of the two sets of replays in flight
+    hipGraph_t graph[kNumCandidates] = {};
+    hipGraphExec_t exec[kNumCandidates] = {};
+    int chosen = -1;                        // tiling candidate of the non-checking launches
+    double cand_ms[kNumCandidates] = {};    // median replay time of each candidate while tuning
+    int64_t replays = 0;
 };
 
 // Check blocks per replay of each group. Every replay advances each running member by the same number
@@ -80,6 +94,24 @@ int group_sub_blocks(const std::vector<ptd_params>& params, const std::vector<in
 // follows one single-level launch.
 int launches_per_block(int kind, int nout) { return kind == ptd::kSweep2D ? nout / 2 + 1 : nout / 2; }
 
+// Estimated work of a member: cells times PT iterations, which grow with the grid's extent (the reference's
+// published counts: about 6 n iterations over 5 steps for linear and step D, 2 n for D = H^3).
+double member_work(const ptd_params& p) {
+    const double n = (double)std::max<int64_t>(p.nx, std::max<int64_t>(p.ny, p.ndim == 3 ? p.nz : 0));
+    const double cells = (double)p.nx * (double)p.ny * (double)(p.ndim == 3 ? p.nz : 1);
+    return cells * n * (p.dcase == PTD_NONLIN ? 2.0 : 6.0);
+}
+
+// Streams of the groups. A device runs a few hardware queues (4 by default) and streams beyond them share
+// one, where the kernels of different streams run in submission order; so the groups are packed onto
+// kLanes streams, the largest estimated work first onto the least loaded lane (PTD_BATCH_LANES overrides).
+constexpr int kLanes = 4;
+int batch_lanes(int ngroups) {
+    const char* e = getenv("PTD_BATCH_LANES");
+    const int want = e && atoi(e) > 0 ? atoi(e) : kLanes;
+    return std::max(1, std::min(want, ngroups));
+}
+
 }  // namespace
 
 struct ptd_batch {
@@ -87,8 +119,8 @@ struct ptd_batch {
     int n = 0;
     int nout = 0;
     hipStream_t stream = nullptr;
-    hipEvent_t ev_fork = nullptr, ev_a = nullptr, ev_b = nullptr;
-    std::vector<hipEvent_t> ev_join;
+    hipEvent_t ev_fork = nullptr;
+    std::vector<hipStream_t> lanes;  // the groups' streams (several groups may share one)
     std::vector<ptd_params> params;
     std::vector<BatchMember> hm;
     BatchMember* dm = nullptr;
@@ -97,13 +129,9 @@ struct ptd_batch {
     int64_t* d_iters = nullptr;
     double* d_errs = nullptr;
     int nt_cap = 0;
-    int32_t* d_active = nullptr;
-    int32_t* h_active = nullptr;
+    int32_t* h_active = nullptr;  // pinned, one slot per group
     uint64_t* d_t0 = nullptr;
-    hipGraph_t graph[kNumCandidates] = {};
-    hipGraphExec_t exec[kNumCandidates] = {};
-    int chosen = -1;
-    double cand_ms[kNumCandidates] = {};
+    bool graphs = false;
     double wall_khz = 0;
     int64_t bytes = 0;
 };
@@ -143,12 +171,14 @@ void release(ptd_batch* b, void* ptr, size_t bytes) {
 }
 
 void destroy_graphs(ptd_batch* b) {
-    for (int c = 0; c < kNumCandidates; ++c) {
-        if (b->exec[c]) (void)hipGraphExecDestroy(b->exec[c]);
-        if (b->graph[c]) (void)hipGraphDestroy(b->graph[c]);
-        b->exec[c] = nullptr;
-        b->graph[c] = nullptr;
-    }
+    for (auto& g : b->groups)
+        for (int c = 0; c < kNumCandidates; ++c) {
+            if (g.exec[c]) (void)hipGraphExecDestroy(g.exec[c]);
+            if (g.graph[c]) (void)hipGraphDestroy(g.graph[c]);
+            g.exec[c] = nullptr;
+            g.graph[c] = nullptr;
+        }
+    b->graphs = false;
 }
 
 void free_batch(ptd_batch* b) {
@@ -158,12 +188,17 @@ void free_batch(ptd_batch* b) {
     destroy_graphs(b);
     for (void* p : b->allocs) (void)hipFree(p);
     if (b->h_active) (void)hipHostFree(b->h_active);
-    for (auto& g : b->groups)
-        if (g.stream) (void)hipStreamDestroy(g.stream);
-    for (auto e : b->ev_join) (void)hipEventDestroy(e);
+    for (auto l : b->lanes) {
+        (void)hipStreamSynchronize(l);
+        (void)hipStreamDestroy(l);
+    }
+    for (auto& g : b->groups) {
+        if (g.ev_a) (void)hipEventDestroy(g.ev_a);
+        if (g.ev_b) (void)hipEventDestroy(g.ev_b);
+        for (auto e : g.ev_poll)
+            if (e) (void)hipEventDestroy(e);
+    }
     if (b->ev_fork) (void)hipEventDestroy(b->ev_fork);
-    if (b->ev_a) (void)hipEventDestroy(b->ev_a);
-    if (b->ev_b) (void)hipEventDestroy(b->ev_b);
     if (b->stream) (void)hipStreamDestroy(b->stream);
     delete b;
 }
@@ -191,62 +226,56 @@ int upload_table(ptd_batch* b, const std::vector<BatchTile>& t, BatchTile** d) {
     return PTD_OK;
 }
 
-// one replay: every group's check blocks of nout iterations on its own stream (each but the last closed
-// by the group's own controller and Hold .= H), then reduction, the controller of every member,
-// Hold .= H and the running-member count to the host
-int enqueue_block(ptd_batch* b, int cand) {
-    PTD_HIPB(hipEventRecord(b->ev_fork, b->stream));
-    for (size_t gi = 0; gi < b->groups.size(); ++gi) {
-        Group& g = b->groups[gi];
-        const int nm = (int)g.members.size();
-        PTD_HIPB(hipStreamWaitEvent(g.stream, b->ev_fork, 0));
-        const int lpb = launches_per_block(g.kind, b->nout);
-        for (int sb = 0; sb < g.sub; ++sb) {
-            if (sb > 0) {
-                PTD_HIPB(ptd::launch_batch_control_list(g.stream, b->dm, g.d_members, nm, b->d_iters, b->d_errs,
-                                                        b->nt_cap));
-                PTD_HIPB(ptd::launch_batch_assign_list(g.stream, b->dm, g.d_members, nm));
+// one replay of group gi on its own stream: its check blocks of nout iterations, each closed by the
+// group's controller and Hold .= H, then its running-member count to the host
+int enqueue_group(ptd_batch* b, size_t gi, int cand) {
+    Group& g = b->groups[gi];
+    const int nm = (int)g.members.size();
+    PTD_HIPB(ptd::launch_batch_span(g.stream, g.d_span, false));
+    const int lpb = launches_per_block(g.kind, b->nout);
+    for (int sb = 0; sb < g.sub; ++sb) {
+        for (int li = 0; li < lpb; ++li) {
+            const bool check = li == lpb - 1;
+            if (g.kind == ptd::kSweep2D && li < lpb - 2) {  // a two-level pair
+                PTD_HIPB(ptd::launch_batch_sweep(g.stream, ptd::kSweep2D2, g.dcase, g.arith,
+                                                 ptd::batch_rows_nocheck(ptd::kSweep2D2, g.dcase, g.arith, 0),
+                                                 false, b->dm, g.d_t2, g.n_t2, li));
+                continue;
             }
-            for (int li = 0; li < lpb; ++li) {
-                const bool check = li == lpb - 1;
-                if (g.kind == ptd::kSweep2D && li < lpb - 2) {  // a two-level pair
-                    PTD_HIPB(ptd::launch_batch_sweep(g.stream, ptd::kSweep2D2, g.dcase, g.arith,
-                                                     ptd::batch_rows_nocheck(ptd::kSweep2D2, g.dcase, g.arith, 0),
-                                                     false, b->dm, g.d_t2, g.n_t2, li));
-                    continue;
-                }
-                const int rows = check ? ptd::batch_rows_check(g.kind, g.dcase, g.arith)
-                                       : ptd::batch_rows_nocheck(g.kind, g.dcase, g.arith, kCandidates[cand].rows2d);
-                PTD_HIPB(ptd::launch_batch_sweep(g.stream, g.kind, g.dcase, g.arith, rows, check, b->dm,
-                                                 check ? g.d_check : g.d_nc[cand], check ? g.n_check : g.n_nc[cand],
-                                                 li));
-            }
-            PTD_HIPB(ptd::launch_batch_reduce(g.stream, b->dm, g.d_members, nm));
+            const int rows = check ? ptd::batch_rows_check(g.kind, g.dcase, g.arith)
+                                   : ptd::batch_rows_nocheck(g.kind, g.dcase, g.arith, kCandidates[cand].rows2d);
+            PTD_HIPB(ptd::launch_batch_sweep(g.stream, g.kind, g.dcase, g.arith, rows, check, b->dm,
+                                             check ? g.d_check : g.d_nc[cand], check ? g.n_check : g.n_nc[cand], li));
         }
-        PTD_HIPB(hipEventRecord(b->ev_join[gi], g.stream));
-        PTD_HIPB(hipStreamWaitEvent(b->stream, b->ev_join[gi], 0));
+        PTD_HIPB(ptd::launch_batch_reduce(g.stream, b->dm, g.d_members, nm));
+        PTD_HIPB(ptd::launch_batch_control_list(g.stream, b->dm, g.d_members, nm, b->d_iters, b->d_errs, g.d_active,
+                                                b->nt_cap));
+        PTD_HIPB(ptd::launch_batch_assign_list(g.stream, b->dm, g.d_members, nm));
     }
-    PTD_HIPB(ptd::launch_batch_control(b->stream, b->dm, b->n, b->d_iters, b->d_errs, b->d_active, b->nt_cap));
-    PTD_HIPB(ptd::launch_batch_assign(b->stream, b->dm, b->n));
-    PTD_HIPB(hipMemcpyAsync(b->h_active, b->d_active, sizeof(int32_t), hipMemcpyDeviceToHost, b->stream));
+    PTD_HIPB(ptd::launch_batch_span(g.stream, g.d_span, true));
+    PTD_HIPB(hipMemcpyAsync(g.h_active, g.d_active, sizeof(int32_t), hipMemcpyDeviceToHost, g.stream));
     return PTD_OK;
 }
 
 int build_graphs(ptd_batch* b) {
     destroy_graphs(b);
-    for (int c = 0; c < kNumCandidates; ++c) {
-        PTD_HIPB(hipStreamBeginCapture(b->stream, hipStreamCaptureModeThreadLocal));
-        const int st = enqueue_block(b, c);
-        hipGraph_t gr = nullptr;
-        const hipError_t e = hipStreamEndCapture(b->stream, &gr);
-        if (st != PTD_OK) {
-            if (gr) (void)hipGraphDestroy(gr);
-            return st;
+    for (size_t gi = 0; gi < b->groups.size(); ++gi) {
+        Group& g = b->groups[gi];
+        for (int c = 0; c < kNumCandidates; ++c) {
+            PTD_HIPB(hipStreamBeginCapture(g.stream, hipStreamCaptureModeThreadLocal));
+            const int st = enqueue_group(b, gi, c);
+            hipGraph_t gr = nullptr;
+            const hipError_t e = hipStreamEndCapture(g.stream, &gr);
+            if (st != PTD_OK) {
+                if (gr) (void)hipGraphDestroy(gr);
+                return st;
+            }
+            PTD_HIPB(e);
+            g.graph[c] = gr;
+            PTD_HIPB(hipGraphInstantiate(&g.exec[c], gr, nullptr, nullptr, 0));
         }
-        PTD_HIPB(e);
-        b->graph[c] = gr;
-        PTD_HIPB(hipGraphInstantiate(&b->exec[c], gr, nullptr, nullptr, 0));
     }
+    b->graphs = true;
     return PTD_OK;
 }
 
@@ -310,8 +339,7 @@ int32_t ptd_batch_create(const ptd_params* params, int32_t n, int32_t device, pt
         return st;
     };
     if (hipSetDevice(device) != hipSuccess || hipStreamCreate(&b->stream) != hipSuccess ||
-        hipEventCreateWithFlags(&b->ev_fork, hipEventDisableTiming) != hipSuccess ||
-        hipEventCreate(&b->ev_a) != hipSuccess || hipEventCreate(&b->ev_b) != hipSuccess)
+        hipEventCreateWithFlags(&b->ev_fork, hipEventDisableTiming) != hipSuccess)
         return fail_free(ptd::set_error(PTD_ERR_HIP, "stream / event creation failed"));
     int rate = 0;
     if (hipDeviceGetAttribute(&rate, hipDeviceAttributeWallClockRate, device) != hipSuccess || rate <= 0)
@@ -364,13 +392,34 @@ int32_t ptd_batch_create(const ptd_params* params, int32_t n, int32_t device, pt
         g->members.push_back(m);
     }
     for (auto& g : b->groups) g.sub = group_sub_blocks(b->params, g.members);
+    {
+        const int nl = batch_lanes((int)b->groups.size());
+        for (int l = 0; l < nl; ++l) {
+            hipStream_t st = nullptr;
+            if (hipStreamCreate(&st) != hipSuccess) return fail_free(ptd::set_error(PTD_ERR_HIP, "stream creation failed"));
+            b->lanes.push_back(st);
+        }
+        std::vector<double> work(b->groups.size(), 0.0), load((size_t)nl, 0.0);
+        std::vector<size_t> order(b->groups.size());
+        for (size_t gi = 0; gi < b->groups.size(); ++gi) {
+            order[gi] = gi;
+            for (int32_t m : b->groups[gi].members) work[gi] += member_work(params[m]);
+        }
+        std::sort(order.begin(), order.end(), [&](size_t a, size_t c) { return work[a] > work[c]; });
+        for (size_t gi : order) {
+            const size_t l = (size_t)(std::min_element(load.begin(), load.end()) - load.begin());
+            load[l] += work[gi];
+            b->groups[gi].stream = b->lanes[l];
+        }
+    }
     for (auto& g : b->groups) {
-        if (hipStreamCreate(&g.stream) != hipSuccess) return fail_free(ptd::set_error(PTD_ERR_HIP, "stream creation failed"));
-        hipEvent_t e;
-        if (hipEventCreateWithFlags(&e, hipEventDisableTiming) != hipSuccess)
+        if (hipEventCreate(&g.ev_a) != hipSuccess || hipEventCreate(&g.ev_b) != hipSuccess ||
+            hipEventCreateWithFlags(&g.ev_poll[0], hipEventDisableTiming) != hipSuccess ||
+            hipEventCreateWithFlags(&g.ev_poll[1], hipEventDisableTiming) != hipSuccess)
             return fail_free(ptd::set_error(PTD_ERR_HIP, "event creation failed"));
-        b->ev_join.push_back(e);
         int st = balloc(b, (void**)&g.d_members, g.members.size() * sizeof(int32_t));
+        if (st == PTD_OK) st = balloc(b, (void**)&g.d_span, 2 * sizeof(uint64_t));
+        if (st == PTD_OK) st = balloc(b, (void**)&g.d_active, sizeof(int32_t));
         if (st != PTD_OK) return fail_free(st);
         if (hipMemcpyAsync(g.d_members, g.members.data(), g.members.size() * sizeof(int32_t), hipMemcpyHostToDevice,
                            b->stream) != hipSuccess)
@@ -413,11 +462,11 @@ int32_t ptd_batch_create(const ptd_params* params, int32_t n, int32_t device, pt
         }
     }
     int st = balloc(b, (void**)&b->dm, (size_t)n * sizeof(BatchMember));
-    if (st == PTD_OK) st = balloc(b, (void**)&b->d_active, sizeof(int32_t));
     if (st == PTD_OK) st = balloc(b, (void**)&b->d_t0, sizeof(uint64_t));
     if (st != PTD_OK) return fail_free(st);
-    if (hipHostMalloc((void**)&b->h_active, sizeof(int32_t), hipHostMallocDefault) != hipSuccess)
+    if (hipHostMalloc((void**)&b->h_active, b->groups.size() * sizeof(int32_t), hipHostMallocDefault) != hipSuccess)
         return fail_free(ptd::set_error(PTD_ERR_NOMEM, "pinned host allocation failed"));
+    for (size_t gi = 0; gi < b->groups.size(); ++gi) b->groups[gi].h_active = b->h_active + gi;
     if (hipStreamSynchronize(b->stream) != hipSuccess)
         return fail_free(ptd::set_error(PTD_ERR_HIP, "batch setup failed"));
     *out = b;
@@ -492,8 +541,9 @@ int32_t ptd_batch_solve(ptd_batch* b, int32_t nt, int64_t* iters, double* errs, 
         M.t_done = 0;
     }
     PTD_HIPB(hipMemcpyAsync(b->dm, b->hm.data(), b->hm.size() * sizeof(BatchMember), hipMemcpyHostToDevice, b->stream));
+    for (auto& g : b->groups) PTD_HIPB(hipMemsetAsync(g.d_span, 0, 2 * sizeof(uint64_t), b->stream));  // eager
     PTD_HIPB(hipStreamSynchronize(b->stream));
-    if (!b->exec[0]) {
+    if (!b->graphs) {
         int st = build_graphs(b);
         if (st != PTD_OK) return st;
     }
@@ -503,34 +553,106 @@ int32_t ptd_batch_solve(ptd_batch* b, int32_t nt, int64_t* iters, double* errs, 
 
     const auto t0 = std::chrono::steady_clock::now();
     PTD_HIPB(ptd::launch_batch_clock(b->stream, b->d_t0));
-    int64_t blocks = 0;
-    *b->h_active = b->n;
-    if (b->chosen < 0) {
-        // online tiling choice: one untimed replay, then one timed replay per candidate
-        PTD_HIPB(hipGraphLaunch(b->exec[0], b->stream));
-        PTD_HIPB(hipStreamSynchronize(b->stream));
-        ++blocks;
-        int best = 0;
-        for (int c = 0; c < kNumCandidates && *b->h_active > 0; ++c) {
-            PTD_HIPB(hipEventRecord(b->ev_a, b->stream));
-            PTD_HIPB(hipGraphLaunch(b->exec[c], b->stream));
-            PTD_HIPB(hipEventRecord(b->ev_b, b->stream));
-            PTD_HIPB(hipEventSynchronize(b->ev_b));
-            float ms = 0;
-            PTD_HIPB(hipEventElapsedTime(&ms, b->ev_a, b->ev_b));
-            b->cand_ms[c] = ms;
-            if (ms < b->cand_ms[best]) best = c;
-            ++blocks;
+    PTD_HIPB(hipEventRecord(b->ev_fork, b->stream));
+    const size_t ng = b->groups.size();
+    for (size_t gi = 0; gi < ng; ++gi) {
+        Group& g = b->groups[gi];
+        PTD_HIPB(hipStreamWaitEvent(g.stream, b->ev_fork, 0));  // replays start after the clock stamp
+        *g.h_active = (int32_t)g.members.size();
+        g.replays = 0;
+    }
+    auto replay = [&](Group& g, int cand) -> int {
+        PTD_HIPB(hipGraphLaunch(g.exec[cand], g.stream));
+        ++g.replays;
+        return PTD_OK;
+    };
+    auto sync_all = [&]() -> int {
+        for (auto& g : b->groups) PTD_HIPB(hipStreamSynchronize(g.stream));
+        return PTD_OK;
+    };
+    bool tune = false;
+    for (auto& g : b->groups) tune = tune || g.chosen < 0;
+    if (tune) {
+        // online tiling choice per group: one untimed replay, then kTuneRounds rounds over the candidates in
+        // turn (so that members finishing during the tuning weigh on every candidate alike); the least
+        // median replay time wins. The groups' replays run concurrently, as they do afterwards.
+        constexpr int kTuneRounds = 3;
+        for (auto& g : b->groups)
+            if (g.chosen < 0 && replay(g, 0) != PTD_OK) return PTD_ERR_HIP;
+        if (sync_all() != PTD_OK) return PTD_ERR_HIP;
+        std::vector<std::vector<double>> ms_of(ng * kNumCandidates);
+        for (int r = 0; r < kTuneRounds; ++r)
+            for (int c = 0; c < kNumCandidates; ++c) {
+                for (size_t gi = 0; gi < ng; ++gi) {
+                    Group& g = b->groups[gi];
+                    if (g.chosen >= 0 || *g.h_active == 0) continue;
+                    PTD_HIPB(hipEventRecord(g.ev_a, g.stream));
+                    if (replay(g, c) != PTD_OK) return PTD_ERR_HIP;
+                    PTD_HIPB(hipEventRecord(g.ev_b, g.stream));
+                }
+                for (size_t gi = 0; gi < ng; ++gi) {
+                    Group& g = b->groups[gi];
+                    if (g.chosen >= 0 || g.replays < 2 + r * kNumCandidates + c) continue;
+                    PTD_HIPB(hipEventSynchronize(g.ev_b));
+                    float ms = 0;
+                    PTD_HIPB(hipEventElapsedTime(&ms, g.ev_a, g.ev_b));
+                    ms_of[gi * kNumCandidates + c].push_back(ms);
+                }
+            }
+        for (size_t gi = 0; gi < ng; ++gi) {
+            Group& g = b->groups[gi];
+            if (g.chosen >= 0) continue;
+            int best = 0;
+            for (int c = 0; c < kNumCandidates; ++c) {
+                auto& v = ms_of[gi * kNumCandidates + c];
+                std::sort(v.begin(), v.end());
+                g.cand_ms[c] = v.empty() ? HUGE_VAL : v[v.size() / 2];
+                if (g.cand_ms[c] < g.cand_ms[best]) best = c;
+            }
+            g.chosen = best;
         }
-        b->chosen = best;
     }
+    // every running group replays on its own stream, two sets of kReplaysPerPoll replays in flight: when
+    // the older set has ended the host reads the group's running count (written by each replay) and, while
+    // members run, queues another set, so no group waits for another one or for the host
     constexpr int kReplaysPerPoll = 4;
-    while (*b->h_active > 0) {
-        if (blocks > max_blocks) return ptd::set_error(PTD_ERR_STATE, "batch did not finish within itmax");
-        for (int r = 0; r < kReplaysPerPoll; ++r) PTD_HIPB(hipGraphLaunch(b->exec[b->chosen], b->stream));
-        PTD_HIPB(hipStreamSynchronize(b->stream));
-        blocks += kReplaysPerPoll;
+    if (sync_all() != PTD_OK) return PTD_ERR_HIP;
+    std::vector<int> inflight(ng, 0), oldest(ng, 0);
+    auto queue_set = [&](size_t gi) -> int {
+        Group& g = b->groups[gi];
+        if (g.replays > max_blocks) return ptd::set_error(PTD_ERR_STATE, "batch did not finish within itmax");
+        for (int r = 0; r < kReplaysPerPoll; ++r)
+            if (replay(g, g.chosen) != PTD_OK) return PTD_ERR_HIP;
+        PTD_HIPB(hipEventRecord(g.ev_poll[(oldest[gi] + inflight[gi]) & 1], g.stream));
+        ++inflight[gi];
+        return PTD_OK;
+    };
+    for (size_t gi = 0; gi < ng; ++gi)
+        for (int k = 0; k < 2 && *b->groups[gi].h_active > 0; ++k) {
+            const int st = queue_set(gi);
+            if (st != PTD_OK) return st;
+        }
+    for (;;) {
+        bool any = false, progressed = false;
+        for (size_t gi = 0; gi < ng; ++gi) {
+            if (inflight[gi] == 0) continue;
+            any = true;
+            Group& g = b->groups[gi];
+            const hipError_t q = hipEventQuery(g.ev_poll[oldest[gi]]);
+            if (q == hipErrorNotReady) continue;
+            PTD_HIPB(q);
+            progressed = true;
+            oldest[gi] ^= 1;
+            --inflight[gi];
+            if (*g.h_active > 0 && inflight[gi] < 2) {
+                const int st = queue_set(gi);
+                if (st != PTD_OK) return st;
+            }
+        }
+        if (!any) break;
+        if (!progressed) std::this_thread::sleep_for(std::chrono::microseconds(20));
     }
+    if (sync_all() != PTD_OK) return PTD_ERR_HIP;
     const double wall = std::chrono::duration<double, std::milli>(std::chrono::steady_clock::now() - t0).count();
 
     std::vector<BatchMember> dmh(b->hm.size());
@@ -563,14 +685,46 @@ int32_t ptd_batch_solve(ptd_batch* b, int32_t nt, int64_t* iters, double* errs, 
     return PTD_OK;
 }
 
+int32_t ptd_batch_group_times(const ptd_batch* b, int32_t* ngroups, int32_t* kind, int32_t* dcase, int32_t* members,
+                              double* ms, int32_t cap) {
+    if (!b) return ptd::set_error(PTD_ERR_ARG, "batch is NULL");
+    const int ng = (int)b->groups.size();
+    if (ngroups) *ngroups = ng;
+    if (hipSetDevice(b->device) != hipSuccess) return ptd::set_error(PTD_ERR_HIP, "hipSetDevice failed");
+    for (int g = 0; g < ng && g < cap; ++g) {
+        const Group& G = b->groups[(size_t)g];
+        uint64_t span[2] = {0, 0};
+        if (hipMemcpy(span, G.d_span, sizeof(span), hipMemcpyDeviceToHost) != hipSuccess)
+            return ptd::set_error(PTD_ERR_HIP, "reading the group clocks failed");
+        if (kind) kind[g] = G.kind == ptd::kSweep3D ? 3 : G.kind == ptd::kSweep2D2 ? 22 : 2;
+        if (dcase) dcase[g] = G.dcase;
+        if (members) members[g] = (int32_t)G.members.size();
+        if (ms) ms[g] = b->wall_khz > 0 ? (double)span[1] / b->wall_khz : 0.0;
+    }
+    return PTD_OK;
+}
+
 int32_t ptd_batch_tiling(const ptd_batch* b, int32_t* tz3d, int32_t* rows2d, double* candidate_ms, int32_t ncand) {
     if (!b) return ptd::set_error(PTD_ERR_ARG, "batch is NULL");
-    const int c = b->chosen < 0 ? 0 : b->chosen;
-    if (tz3d) *tz3d = kCandidates[c].tz3d;
-    if (rows2d) *rows2d = kCandidates[c].rows2d == 1 ? 8 : 16;
+    // the choices of the largest 3D group and of the largest single-level 2D group; per candidate the sum
+    // over the groups of their median replay times while tuning
+    const Group *g3 = nullptr, *g2 = nullptr;
+    bool tuned = !b->groups.empty();
+    for (const auto& g : b->groups) {
+        tuned = tuned && g.chosen >= 0;
+        if (g.kind == ptd::kSweep3D && (!g3 || g.members.size() > g3->members.size())) g3 = &g;
+        if (g.kind == ptd::kSweep2D && (!g2 || g.members.size() > g2->members.size())) g2 = &g;
+    }
+    if (tz3d) *tz3d = kCandidates[g3 && g3->chosen >= 0 ? g3->chosen : 0].tz3d;
+    if (rows2d) *rows2d = kCandidates[g2 && g2->chosen >= 0 ? g2->chosen : 0].rows2d == 1 ? 8 : 16;
     if (candidate_ms)
-        for (int k = 0; k < ncand && k < kNumCandidates; ++k) candidate_ms[k] = b->cand_ms[k];
-    return b->chosen < 0 ? PTD_ERR_STATE : PTD_OK;
+        for (int k = 0; k < ncand && k < kNumCandidates; ++k) {
+            double sum = 0;
+            for (const auto& g : b->groups)
+                if (std::isfinite(g.cand_ms[k])) sum += g.cand_ms[k];
+            candidate_ms[k] = sum;
+        }
+    return tuned ? PTD_OK : PTD_ERR_STATE;
 }
 
 }  // extern "C"

@@ -568,12 +568,13 @@ hipError_t launch_batch_sweep(hipStream_t s, int kind, int dcase, int arith, int
 int batch_rows_nocheck(int kind, int dcase, int arith, int variant);  // rows of the non-checking launch
 int batch_rows_check(int kind, int dcase, int arith);                 // rows of the checking launch
 hipError_t launch_batch_reduce(hipStream_t s, BatchMember* mem, const int32_t* members, int n);
-hipError_t launch_batch_control(hipStream_t s, BatchMember* mem, int n, int64_t* iters, double* errs,
-                                int32_t* active_count, int nt_cap);
+// the reference loop's rules for the members of one group after a check block; *active_count (may be null)
+// receives how many of them are still running. Hold .= H for the members whose step ended.
 hipError_t launch_batch_control_list(hipStream_t s, BatchMember* mem, const int32_t* members, int n, int64_t* iters,
-                                     double* errs, int nt_cap);
-hipError_t launch_batch_assign(hipStream_t s, BatchMember* mem, int n);
+                                     double* errs, int32_t* active_count, int nt_cap);
 hipError_t launch_batch_assign_list(hipStream_t s, BatchMember* mem, const int32_t* members, int n);
 hipError_t launch_batch_clock(hipStream_t s, uint64_t* t);
+// span[0] = device clock (start) / span[1] += clock - span[0] (stop)
+hipError_t launch_batch_span(hipStream_t s, uint64_t* span, bool stop);
 
 }  // namespace ptd

@@ -2341,41 +2341,25 @@ __device__ __forceinline__ int control_member(BatchMember& M, int t, int64_t* __
     return M.active;
 }
 
-// Every member, one thread each; the number of members still running goes to *active_count.
-__global__ void __launch_bounds__(kBatchMaxMembers)
-k_batch_control(BatchMember* __restrict__ mem, int n, int64_t* __restrict__ iters, double* __restrict__ errs,
-                int32_t* __restrict__ active_count, int nt_cap) {
-    const int t = threadIdx.x;
-    const int running = t < n ? control_member(mem[t], t, iters, errs, nt_cap) : 0;
-    const int cnt = __syncthreads_count(running);
-    if (t == 0) *active_count = cnt;
-}
-
-hipError_t launch_batch_control(hipStream_t s, BatchMember* mem, int n, int64_t* iters, double* errs,
-                                int32_t* active_count, int nt_cap) {
-    if (n <= 0 || n > kBatchMaxMembers) return hipErrorInvalidValue;
-    hipLaunchKernelGGL(k_batch_control, dim3(1), dim3(kBatchMaxMembers), 0, s, mem, n, iters, errs, active_count,
-                       nt_cap);
-    return hipGetLastError();
-}
-
-// The members of one group (a check block inside a replay: the group runs further check blocks before
-// the replay's closing controller)
+// The members of one group, one thread each (a check block of that group); with `active_count` the number
+// of its members still running goes there.
 __global__ void __launch_bounds__(kBatchMaxMembers)
 k_batch_control_list(BatchMember* __restrict__ mem, const int32_t* __restrict__ members, int n,
-                     int64_t* __restrict__ iters, double* __restrict__ errs, int nt_cap) {
+                     int64_t* __restrict__ iters, double* __restrict__ errs, int32_t* __restrict__ active_count,
+                     int nt_cap) {
     const int t = threadIdx.x;
-    if (t < n) (void)control_member(mem[members[t]], members[t], iters, errs, nt_cap);
+    const int running = t < n ? control_member(mem[members[t]], members[t], iters, errs, nt_cap) : 0;
+    const int cnt = __syncthreads_count(running);
+    if (t == 0 && active_count) *active_count = cnt;
 }
 
 hipError_t launch_batch_control_list(hipStream_t s, BatchMember* mem, const int32_t* members, int n, int64_t* iters,
-                                     double* errs, int nt_cap) {
+                                     double* errs, int32_t* active_count, int nt_cap) {
     if (n <= 0 || n > kBatchMaxMembers) return hipErrorInvalidValue;
     hipLaunchKernelGGL(k_batch_control_list, dim3(1), dim3(kBatchMaxMembers), 0, s, mem, members, n, iters, errs,
-                       nt_cap);
+                       active_count, nt_cap);
     return hipGetLastError();
 }
-
 // Hold .= H for the members whose step just ended (kBatchAssignBlocks blocks per member; `members`
 // lists them, or null for members 0..n-1)
 constexpr int kBatchAssignBlocks = 64;
@@ -2391,12 +2375,6 @@ k_batch_assign(BatchMember* __restrict__ mem, const int32_t* __restrict__ member
         Ho[i] = H[i];
 }
 
-hipError_t launch_batch_assign(hipStream_t s, BatchMember* mem, int n) {
-    if (n <= 0) return hipSuccess;
-    hipLaunchKernelGGL(k_batch_assign, dim3(n * kBatchAssignBlocks), dim3(kBlock), 0, s, mem, nullptr);
-    return hipGetLastError();
-}
-
 hipError_t launch_batch_assign_list(hipStream_t s, BatchMember* mem, const int32_t* members, int n) {
     if (n <= 0) return hipSuccess;
     hipLaunchKernelGGL(k_batch_assign, dim3(n * kBatchAssignBlocks), dim3(kBlock), 0, s, mem, members);
@@ -2409,6 +2387,21 @@ __global__ void k_batch_clock(uint64_t* t) {
 
 hipError_t launch_batch_clock(hipStream_t s, uint64_t* t) {
     hipLaunchKernelGGL(k_batch_clock, dim3(1), dim3(64), 0, s, t);
+    return hipGetLastError();
+}
+
+// span[0] = now (start), or span[1] += now - span[0] (stop): the device time between two points of a
+// stream, summed over graph replays (kernel nodes only: nothing to clear inside the graph)
+__global__ void k_batch_span(uint64_t* span, int stop) {
+    if (threadIdx.x == 0) {
+        const uint64_t now = (uint64_t)wall_clock64();
+        if (stop) span[1] += now - span[0];
+        else span[0] = now;
+    }
+}
+
+hipError_t launch_batch_span(hipStream_t s, uint64_t* span, bool stop) {
+    hipLaunchKernelGGL(k_batch_span, dim3(1), dim3(64), 0, s, span, (int)stop);
     return hipGetLastError();
 }
 

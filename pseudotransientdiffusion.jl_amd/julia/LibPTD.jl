@@ -11,7 +11,8 @@ module LibPTD
 export PtdParams, Context, create, destroy, set_params!, set!, get!, field_count, device_bytes,
        compute_iter_params!, compute_flux!, compute_update!, compute_flux_res!, check_res!, assign!,
        iterate!, iterate_timed!, pt_solve!, solve!, outer_iterations, grad_solve!,
-       Batch, batch_create, batch_destroy, batch_set!, batch_get!, batch_solve!, batch_tiling
+       Batch, batch_create, batch_destroy, batch_set!, batch_get!, batch_solve!, batch_tiling,
+       batch_group_times
 
 const libptd = get(ENV, "LIBPTD", joinpath(@__DIR__, "..", "libptd.so"))
 
@@ -253,6 +254,23 @@ function batch_tiling(b::Batch)
     check(ccall((:ptd_batch_tiling, libptd), Int32, (Ptr{Cvoid}, Ref{Int32}, Ref{Int32}, Ptr{Float64}, Int32),
                 b.handle, tz, rows, ms, length(ms)))
     return tz[], rows[], ms
+end
+
+"""
+    batch_group_times(b) -> Vector of (kind, dcase, members, ms)
+
+Per member group of the last `batch_solve!`: sweep kind (3: 3D two-level, 22: 2D two-level, 2: 2D), D case,
+member count and the device time the group's stream took, summed over the replays.
+"""
+function batch_group_times(b::Batch; cap::Integer=64)
+    ng = Ref(Int32(0))
+    kind, dcase, mem = zeros(Int32, cap), zeros(Int32, cap), zeros(Int32, cap)
+    ms = zeros(Float64, cap)
+    check(ccall((:ptd_batch_group_times, libptd), Int32,
+                (Ptr{Cvoid}, Ref{Int32}, Ptr{Int32}, Ptr{Int32}, Ptr{Int32}, Ptr{Float64}, Int32),
+                b.handle, ng, kind, dcase, mem, ms, cap))
+    n = min(Int(ng[]), cap)
+    return [(kind[g], dcase[g], mem[g], ms[g]) for g in 1:n]
 end
 
 end # module

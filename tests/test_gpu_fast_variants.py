@@ -297,6 +297,7 @@ def test_batch_workload_96(pkg, orc):
             if D is not None:
                 b.set(i, "D", D)
         it, er, lat, st, total = b.solve(5)
+        groups = b.group_times()
         order2 = sorted((m.n, i) for i, m in enumerate(w) if "2D" in m.script)[:2]
         order3 = sorted((m.n, i) for i, m in enumerate(w) if "3D" in m.script)[:2]
         sample = [i for _, i in order2 + order3]
@@ -306,6 +307,9 @@ def test_batch_workload_96(pkg, orc):
     assert np.all(er <= 1e-8)
     assert np.all(it > 0) and np.all(it % NOUT == 0)
     assert np.all(lat > 0) and np.all(lat <= total * 1.01 + 1.0)
+    print(f"[gpu] batch groups: {groups}", flush=True)
+    assert sum(g["members"] for g in groups) == 96
+    assert all(0 < g["ms"] <= total * 1.01 for g in groups)
     for i in sample:
         m = w[i]
         H0, D = B.member_fields(m)
