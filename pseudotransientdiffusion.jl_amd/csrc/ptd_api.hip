@@ -221,14 +221,25 @@ int one_iteration(ptd_ctx* c, bool check) {
 
 // n PT iterations; with `check_last` the last one also leaves the residual reduction in dred. The 3D
 // path runs pairs of iterations through the two-level sweep (one HBM pass per pair) and a single
-// sweep for an odd remainder; PTD_T2=0 in the environment forces single sweeps.
+// sweep for an odd remainder; PTD_T2=0 in the environment forces single sweeps. Small 2D grids run
+// their iterations in LDS-resident launches of up to lds2d_max_iters() each, the last one checking (plain
+// form; the other forms check in a single sweep after them).
 int run_iterations(ptd_ctx* c, int64_t n, bool check_last) {
     const char* t2e = getenv("PTD_T2");
     const bool t2_env = !(t2e && t2e[0] == '0');
     const bool plain = c->p.form == PTD_PLAIN;
     const bool t2 = t2_env && ptd::fused2_available(c);
+    const bool lds = ptd::use_2d_lds(c);
+    const int64_t lmax = ptd::lds2d_max_iters();
     while (n > 0) {
-        if (t2 && n >= 2) {
+        if (lds && check_last && plain && n <= lmax) {  // the checking launch, residual reduced in it
+            PTD_HIP(ptd::launch_lds2d(c, (int)n, true));
+            n = 0;
+        } else if (lds && n - (check_last ? 1 : 0) >= 1) {
+            const int m = (int)std::min<int64_t>(n - (check_last ? 1 : 0), lmax);
+            PTD_HIP(ptd::launch_lds2d(c, m, false));
+            n -= m;
+        } else if (t2 && n >= 2) {
             const bool chk = check_last && n == 2;
             PTD_HIP(ptd::launch_fused2(c, chk && plain));
             if (chk && !plain) {

@@ -486,6 +486,13 @@ hipError_t launch_fused2(ptd_ctx* c, bool check);         // two fused PT iterat
 int64_t max_chunk_tz(const ptd_params& p, int64_t esz);
 bool fused_available(const ptd_ctx* c);
 bool fused2_available(const ptd_ctx* c);
+// 2D: niter (1..lds2d_max_iters()) PT iterations in one LDS-resident launch (check: the last one's
+// plain-form residual reduced into dred), and whether a single-problem 2D solve runs its iterations that
+// way (small grids)
+hipError_t launch_lds2d(ptd_ctx* c, int niter, bool check);
+int lds2d_max_iters();
+bool use_2d_lds(const ptd_ctx* c);
+int64_t lds2d_tiles(const ptd_params& p);
 // residual partials (tiles) the checking launches of a fused 2D / 3D solve write at most
 int64_t fused_check_tiles(const ptd_ctx* c);
 // mixed precision: float32 correction sweeps (1 or 2 iterations), source and correction kernels

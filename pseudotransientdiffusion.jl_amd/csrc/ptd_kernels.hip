@@ -692,6 +692,172 @@ k_fused2d_t2(const KP<T> p, const T* __restrict__ Hin, T* __restrict__ Hout, con
 }
 
 // ---------------------------------------------------------------------------------------------
+// 2D PT iterations resident in LDS (small grids)
+// ---------------------------------------------------------------------------------------------
+//
+// On small 2D grids one PT iteration is a few microseconds of load -> flux -> update latency plus a
+// launch gap, whatever its bandwidth. This sweep runs up to kLdsHalo iterations per launch: a block owns
+// a kLdsCore^2 core of cells and loads the core plus a kLdsHalo-cell frame once. Every iteration's
+// results are exact only on a region one cell narrower per side than the last (a cell needs its
+// neighbours' previous values), so after m <= kLdsHalo iterations the core holds exactly what m single
+// sweeps give; the block then writes its core cells and the faces it owns (the lower x / y face of each
+// core cell) and nothing else. Iteration k computes only the region its remaining iterations need.
+//
+// Frame cell c = y*W + x (W = kLdsCore + 2*kLdsHalo) is global cell (kLdsCore*bx - kLdsHalo + x,
+// kLdsCore*by - kLdsHalo + y); its state (H, Hold, the flux on its lower x and y face, D and the step
+// case's neighbourhood maximum) lives in the registers of thread c % kLdsThreads, slot c / kLdsThreads.
+// An iteration publishes H to LDS, computes each owned cell's lower faces from its west / south
+// neighbour's H and publishes the fluxes, then updates each owned cell from its east / north neighbours'
+// fluxes: the same expressions on the same operands as the single sweep (compute_flux! then
+// compute_update!, scripts/diff_2D_nonlin.jl:18-44), so EXACT arithmetic gives the same bits.
+//
+// CHECK (plain form): the last iteration also evaluates the residual on the core (residual fluxes from
+// the pre-update H, transient term from the updated H: scripts/diff_2D_nonlin.jl:28-37) and every block
+// writes its (sum r^2, max|r|) partial. (Finishing the reduction in the last block to arrive needs an
+// agent-scope fence per block, which writes back the XCD's L2: measured 2-13 ms slower per 512^2 -
+// 1024^2 solve than the separate one-block reduction.)
+static constexpr int kLdsCore = 32;
+static constexpr int kLdsHalo = 10;
+static constexpr int kLdsW = kLdsCore + 2 * kLdsHalo;  // 52
+static constexpr int kLdsThreads = 1024;
+
+template <typename T, int DC, int FORM, int ARITH, bool CHECK>
+__global__ void __launch_bounds__(kLdsThreads)
+k_lds2d(const KP<T> p, const T* __restrict__ Hin, T* __restrict__ Hout, const T* __restrict__ Hold,
+        const T* __restrict__ D, const T* __restrict__ qxi, const T* __restrict__ qyi, T* __restrict__ qxo,
+        T* __restrict__ qyo, int niter, int ntx, int ntiles, int per_xcd, double2* __restrict__ partial) {
+    using Ph = Phys<T, DC, FORM, ARITH>;
+    constexpr bool STEP = DC == PTD_STEP;
+    constexpr int W = kLdsW, NC = W * W;
+    constexpr int NS = (NC + kLdsThreads - 1) / kLdsThreads;
+    __shared__ T sH[NC];
+    __shared__ T sQx[NC];
+    __shared__ T sQy[NC];
+    __shared__ T sRx[CHECK ? NC : 1];  // residual fluxes of the checking iteration
+    __shared__ T sRy[CHECK ? NC : 1];
+
+    const int b = blockIdx.x;
+    const int tile = (b & 7) * per_xcd + (b >> 3);  // XCD-aware: each XCD walks a contiguous tile range
+    if (tile >= ntiles) return;
+    const int nx = (int)p.nx, ny = (int)p.ny;
+    const int x0 = kLdsCore * (tile % ntx) - kLdsHalo;
+    const int y0 = kLdsCore * (tile / ntx) - kLdsHalo;
+    const int t = threadIdx.x;
+
+    T H[NS], Ho[NS], qx[NS], qy[NS], Dc[NS], Dw[NS], Ds[NS], dmax[NS];
+    int xs[NS], ys[NS];
+    bool fxs[NS], fys[NS], upds[NS];
+#pragma unroll
+    for (int s = 0; s < NS; ++s) {
+        const int c = t + kLdsThreads * s;
+        const bool valid = c < NC;
+        const int y = c / W, x = c - y * W;
+        const int i = x0 + x, j = y0 + y;
+        const bool cin = valid && i >= 0 && i < nx && j >= 0 && j < ny;
+        const int64_t o = (int64_t)i + (int64_t)nx * j;
+        xs[s] = valid ? x : -W;  // invalid slots fall outside every active region
+        ys[s] = valid ? y : -W;
+        fxs[s] = valid && i >= 1 && i <= nx - 1 && j >= 1 && j <= ny - 2;
+        fys[s] = valid && j >= 1 && j <= ny - 1 && i >= 1 && i <= nx - 2;
+        upds[s] = valid && i >= 1 && i <= nx - 2 && j >= 1 && j <= ny - 2;
+        H[s] = cin ? Hin[o] : (T)0;
+        Ho[s] = cin ? Hold[o] : (T)0;
+        Dc[s] = (STEP && cin) ? D[o] : (T)0;
+        qx[s] = fxs[s] ? qxi[(int64_t)(i - 1) + (int64_t)(nx - 1) * (j - 1)] : (T)0;
+        qy[s] = fys[s] ? qyi[(int64_t)(i - 1) + (int64_t)(nx - 2) * (j - 1)] : (T)0;
+        Dw[s] = Ds[s] = dmax[s] = (T)0;
+    }
+    if constexpr (STEP) {
+        // D is constant over the launch: neighbours and the neighbourhood maximum once, through sH
+#pragma unroll
+        for (int s = 0; s < NS; ++s)
+            if (t + kLdsThreads * s < NC) sH[t + kLdsThreads * s] = Dc[s];
+        __syncthreads();
+#pragma unroll
+        for (int s = 0; s < NS; ++s) {
+            const int c = t + kLdsThreads * s;
+            if (xs[s] >= 1) Dw[s] = sH[c - 1];  // faces reach column / row W-1
+            if (ys[s] >= 1) Ds[s] = sH[c - W];
+            if (xs[s] >= 1 && xs[s] <= W - 2 && ys[s] >= 1 && ys[s] <= W - 2) {
+                dmax[s] = fmax(fmax(Dw[s], sH[c + 1]), Dc[s]);
+                dmax[s] = fmax(dmax[s], fmax(Ds[s], sH[c + W]));
+            }
+        }
+        __syncthreads();
+    }
+
+    double rs = 0.0, rm = 0.0;  // residual (sum r^2, max|r|) of this thread's core cells
+    for (int k = 1; k <= niter; ++k) {
+        const bool chk = CHECK && k == niter;
+        // after this iteration cells [lo, W-1-lo] and faces [lo, W-lo] along each axis must be exact
+        const int lo = k + kLdsHalo - niter;
+#pragma unroll
+        for (int s = 0; s < NS; ++s)
+            if (t + kLdsThreads * s < NC) sH[t + kLdsThreads * s] = H[s];
+        __syncthreads();
+#pragma unroll
+        for (int s = 0; s < NS; ++s) {
+            const int c = t + kLdsThreads * s;
+            const bool yrow = ys[s] >= lo && ys[s] <= W - 1 - lo, xcol = xs[s] >= lo && xs[s] <= W - 1 - lo;
+            T rx = 0, ry = 0;
+            // qx faces [lo, W-lo] of rows [lo, W-1-lo]; qy faces [lo, W-lo] of columns [lo, W-1-lo]
+            if (fxs[s] && yrow && xs[s] >= lo && xs[s] <= W - lo) {
+                if (chk)
+                    qx[s] = Ph::template face<true>(p, qx[s], sH[c - 1], H[s], Dw[s], Dc[s], p.dx, p._dx, p._dx8, rx);
+                else
+                    qx[s] = Ph::template face<false>(p, qx[s], sH[c - 1], H[s], Dw[s], Dc[s], p.dx, p._dx, p._dx8, rx);
+            }
+            if (fys[s] && xcol && ys[s] >= lo && ys[s] <= W - lo) {
+                if (chk)
+                    qy[s] = Ph::template face<true>(p, qy[s], sH[c - W], H[s], Ds[s], Dc[s], p.dy, p._dy, p._dy8, ry);
+                else
+                    qy[s] = Ph::template face<false>(p, qy[s], sH[c - W], H[s], Ds[s], Dc[s], p.dy, p._dy, p._dy8, ry);
+            }
+            if (c < NC) {
+                sQx[c] = qx[s];
+                sQy[c] = qy[s];
+                if constexpr (CHECK) {
+                    if (chk) {
+                        sRx[c] = rx;
+                        sRy[c] = ry;
+                    }
+                }
+            }
+        }
+        __syncthreads();
+#pragma unroll
+        for (int s = 0; s < NS; ++s) {
+            const int c = t + kLdsThreads * s;
+            if (upds[s] && xs[s] >= lo && xs[s] <= W - 1 - lo && ys[s] >= lo && ys[s] <= W - 1 - lo) {
+                const T dv = Ph::template div<2>(p, sQx[c + 1] - qx[s], sQy[c + W] - qy[s], (T)0);
+                H[s] = Ph::cell(p, H[s], Ho[s], dv, dmax[s], Dc[s]);
+                if constexpr (CHECK) {
+                    if (chk) {  // lo == kLdsHalo: exactly the core
+                        const T dv2 = Ph::template div<2>(p, sRx[c + 1] - sRx[c], sRy[c + W] - sRy[c], (T)0);
+                        const double r = (double)Ph::resid(p, H[s], Ho[s], dv2);
+                        rs += r * r;
+                        rm = nanmax(fabs(r), rm);
+                    }
+                }
+            }
+        }
+    }
+
+    // the core: inner cells and the lower faces of its cells
+#pragma unroll
+    for (int s = 0; s < NS; ++s) {
+        const int x = xs[s], y = ys[s];
+        if (x < kLdsHalo || x >= kLdsHalo + kLdsCore || y < kLdsHalo || y >= kLdsHalo + kLdsCore) continue;
+        const int i = x0 + x, j = y0 + y;
+        if (upds[s]) Hout[(int64_t)i + (int64_t)nx * j] = H[s];
+        if (fxs[s]) qxo[(int64_t)(i - 1) + (int64_t)(nx - 1) * (j - 1)] = qx[s];
+        if (fys[s]) qyo[(int64_t)(i - 1) + (int64_t)(nx - 2) * (j - 1)] = qy[s];
+    }
+
+    if constexpr (CHECK) block_sum_max(rs, rm, t, kLdsThreads, &partial[tile]);
+}
+
+// ---------------------------------------------------------------------------------------------
 // fused 3D sweep over two PT iterations (temporal blocking)
 // ---------------------------------------------------------------------------------------------
 //
@@ -1302,6 +1468,31 @@ static hipError_t sweep2d_t2_run(ptd_ctx* c, const SweepBufs& b, bool check) {
     return hipGetLastError();
 }
 
+// niter (1..kLdsHalo) 2D iterations in one LDS-resident launch; with `check` the plain-form residual of
+// the last one is reduced into c->dred.
+template <typename T, int DC, int FORM, int ARITH>
+static hipError_t sweep2d_lds_run(ptd_ctx* c, const SweepBufs& b, int niter, bool check) {
+    const KP<T> kp = make_kp<T>(c->p);
+    const int ntx = (int)((c->p.nx + kLdsCore - 1) / kLdsCore);
+    const int nty = (int)((c->p.ny + kLdsCore - 1) / kLdsCore);
+    const int ntiles = ntx * nty;
+    const int per = (ntiles + 7) / 8;
+    double2* part = (double2*)c->partial;
+    if (check)
+        hipLaunchKernelGGL((k_lds2d<T, DC, FORM, ARITH, true>), dim3(8 * per), dim3(kLdsThreads), 0, c->stream, kp,
+                           (const T*)b.Hin, (T*)b.Hout, (const T*)b.Hold, (const T*)b.aux, (const T*)b.qin[0],
+                           (const T*)b.qin[1], (T*)b.qout[0], (T*)b.qout[1], niter, ntx, ntiles, per, part);
+    else
+        hipLaunchKernelGGL((k_lds2d<T, DC, FORM, ARITH, false>), dim3(8 * per), dim3(kLdsThreads), 0, c->stream, kp,
+                           (const T*)b.Hin, (T*)b.Hout, (const T*)b.Hold, (const T*)b.aux, (const T*)b.qin[0],
+                           (const T*)b.qin[1], (T*)b.qout[0], (T*)b.qout[1], niter, ntx, ntiles, per, part);
+    hipError_t e = hipGetLastError();
+    if (e != hipSuccess || !check) return e;
+    hipLaunchKernelGGL(k_reduce_partials, dim3(1), dim3(kBlock), 0, c->stream, (const double2*)part, (int64_t)ntiles,
+                       c->dred);
+    return hipGetLastError();
+}
+
 // The context's own fields (H, q double-buffered; the sweep flips them).
 static SweepBufs ctx_bufs(ptd_ctx* c) {
     const int h = c->hcur, q = c->qcur;
@@ -1340,6 +1531,16 @@ template <typename T, int DC, int FORM>
 static hipError_t fused_t(ptd_ctx* c, bool check) { return ctx_sweep_t<T, DC, FORM, false>(c, check); }
 template <typename T, int DC, int FORM>
 static hipError_t fused2_t(ptd_ctx* c, bool check) { return ctx_sweep_t<T, DC, FORM, true>(c, check); }
+template <typename T, int DC, int FORM>
+static hipError_t lds2d_t(ptd_ctx* c, int niter, bool check) {
+    const SweepBufs b = ctx_bufs(c);
+    const hipError_t e = c->p.arith == PTD_ARITH_FAST ? sweep2d_lds_run<T, DC, FORM, kFast>(c, b, niter, check)
+                                                      : sweep2d_lds_run<T, DC, FORM, kExact>(c, b, niter, check);
+    if (e != hipSuccess) return e;
+    c->hcur = 1 - c->hcur;
+    c->qcur = 1 - c->qcur;
+    return hipSuccess;
+}
 
 // dispatch on (precision, dcase, form)
 #define PTD_DISPATCH(FN, c, ...)                                                                        \
@@ -1366,6 +1567,12 @@ hipError_t launch_flux(ptd_ctx* c, int q2mode) { PTD_DISPATCH(flux_t, c, q2mode)
 hipError_t launch_update(ptd_ctx* c) { PTD_DISPATCH(update_t, c); }
 hipError_t launch_fused(ptd_ctx* c, bool check) { PTD_DISPATCH(fused_t, c, check); }
 hipError_t launch_fused2(ptd_ctx* c, bool check) { PTD_DISPATCH(fused2_t, c, check); }
+hipError_t launch_lds2d(ptd_ctx* c, int niter, bool check) {
+    if (niter < 1 || niter > kLdsHalo || c->p.ndim != 2 || (check && c->p.form != PTD_PLAIN))
+        return hipErrorInvalidValue;
+    PTD_DISPATCH(lds2d_t, c, niter, check);
+}
+int lds2d_max_iters() { return kLdsHalo; }
 
 hipError_t launch_residual(ptd_ctx* c, bool write_resh) {
     const bool f32 = c->esz == 4;
@@ -1398,6 +1605,21 @@ bool use_2d_t2(const ptd_params& p) {
     return ntx * nty >= 1024;
 }
 int batch_kind(const ptd_params& p) { return p.ndim == 3 ? kSweep3D : (use_2d_t2(p) ? kSweep2D2 : kSweep2D); }
+
+// LDS-resident 2D iterations pay off while a launch of single sweeps is latency-bound: grids of at most
+// kLdsMaxTiles 32x32 cores. PTD_LDS2D=0 / 1 in the environment switches them off / on for every 2D grid.
+static constexpr int64_t kLdsMaxTiles = 1024;
+int64_t lds2d_tiles(const ptd_params& p) {
+    return ((p.nx + kLdsCore - 1) / kLdsCore) * ((p.ny + kLdsCore - 1) / kLdsCore);
+}
+bool use_2d_lds(const ptd_ctx* c) {
+    const ptd_params& p = c->p;
+    if (p.ndim != 2 || !fused_available(c)) return false;
+    const char* env = getenv("PTD_LDS2D");
+    if (env && env[0] == '0') return false;
+    if (env && env[0] == '1') return true;
+    return lds2d_tiles(p) <= kLdsMaxTiles;
+}
 // PTD_T2=2 in the environment also runs the single-problem 2D solves on two-level sweeps below that
 // threshold (the same bits as single sweeps; lets the tests compare them with the oracle on small grids).
 // Batches keep the threshold.
@@ -1767,7 +1989,7 @@ int64_t fused_check_tiles(const ptd_ctx* c) {
     if (p.ndim == 2) {
         const int64_t ntx = (p.nx - 2 + 61) / 62, nty = (p.ny - 2 + kRows2D - 3) / (kRows2D - 2);
         const int64_t ntx2 = (p.nx - 2 + 59) / 60, nty2 = (p.ny - 2 + kTy2D - 1) / kTy2D;
-        return std::max<int64_t>(ntx * nty, ntx2 * nty2);
+        return std::max<int64_t>(std::max<int64_t>(ntx * nty, ntx2 * nty2), lds2d_tiles(p));
     }
     const int64_t tz = fused_tz(c, true);
     const int64_t nbz = (p.nz - 2 + tz - 1) / tz;
@@ -1783,6 +2005,7 @@ int64_t partial_capacity(const ptd_params& p, int64_t nR) {
         nb = std::max<int64_t>(nb, ntx * nty);
         const int64_t ntx2 = (p.nx - 2 + 59) / 60, nty2 = (p.ny - 2 + kTy2D - 1) / kTy2D;
         nb = std::max<int64_t>(nb, ntx2 * nty2);
+        nb = std::max<int64_t>(nb, lds2d_tiles(p));
     }
     if (p.ndim == 3) {
         const int64_t ntx = (p.nx - 2 + 61) / 62, nty = (p.ny - 2 + kRows - 3) / (kRows - 2);

@@ -88,6 +88,7 @@ def test_fast_2d_sweeps_close_to_oracle(script, shape, t2, pkg, orc, monkeypatch
     the fluxes within 1e-12 relative max-norm of the oracle's 5 iterations."""
     print(f"[gpu] fast 2D sweeps {script} {shape} PTD_T2={t2}", flush=True)
     monkeypatch.setenv("PTD_T2", t2)
+    monkeypatch.setenv("PTD_LDS2D", "0")  # the sweeps named by PTD_T2 (the LDS iterations have their own tests)
     params, case, H, Hold, D, q = _random_state(pkg, orc, script, shape, seed=sum(shape) + 3)
     Hg, qg = _gpu_iterate(pkg, params, H, Hold, D, q, 5)
     Hr, qr = _oracle_iterate(orc, pkg, params, case, H, Hold, D, q, 5)
@@ -106,6 +107,7 @@ def test_fast_2d_solve_matches_oracle(script, t2, pkg, orc, monkeypatch):
     shape = (96, 80)
     print(f"[gpu] fast 2D solve {script} {shape} PTD_T2={t2}", flush=True)
     monkeypatch.setenv("PTD_T2", t2)
+    monkeypatch.setenv("PTD_LDS2D", "0")
     over_o = dict(nout=NOUT, itmax=100000) if "perf" in script else {}
     over_g = dict(nout=NOUT, itMax=1e5) if "perf" in script else {}
     ref = orc.run_script(script, *shape, **over_o)

@@ -382,6 +382,7 @@ def test_two_level_2d_sweeps_match_oracle(script, pkg, orc, monkeypatch):
     niter = 7
     print(f"[gpu] t2 2D {script} {T2D_SHAPE} niter={niter}", flush=True)
     monkeypatch.setenv("PTD_T2", "1")
+    monkeypatch.setenv("PTD_LDS2D", "0")  # the two-level sweeps, not the LDS-resident iterations
     params, case, H, Hold, D, q = _random_state(pkg, orc, script, T2D_SHAPE, seed=11)
     with pkg.Context(params) as ctx:
         ctx.set("H", H)
@@ -415,6 +416,7 @@ def test_two_level_2d_converged_step_equals_single_level(script, precision, pkg,
     print(f"[gpu] t2 2D converged step {script} precision={precision}", flush=True)
     spec = pkg.SCRIPTS[script]
     out = []
+    monkeypatch.setenv("PTD_LDS2D", "0")
     for t2 in ("1", "0"):
         monkeypatch.setenv("PTD_T2", t2)
         params = pkg.make_params(spec, *T2D_SHAPE, precision=precision, arith=1)
