@@ -140,6 +140,8 @@ void free_ctx(ptd_ctx* c) {
     if (c->hred) (void)hipHostFree(c->hred);
     if (c->ev0) (void)hipEventDestroy(c->ev0);
     if (c->ev1) (void)hipEventDestroy(c->ev1);
+    for (int k = 0; k < 2; ++k)
+        if (c->evc[k]) (void)hipEventDestroy(c->evc[k]);
     if (c->stream) (void)hipStreamDestroy(c->stream);
     delete c;
 }
@@ -387,11 +389,74 @@ int pt_solve_mixed(ptd_ctx* c, int64_t* iters, double* err_rms, double* err_max)
     return PTD_OK;
 }
 
+// The solve loop below for small 2D grids, whose check blocks are one LDS-resident launch each (plain
+// form, nout <= lds2d_max_iters()) and last a few tens of microseconds: the next block is enqueued before
+// the previous block's residual is read, so the GPU does not wait for the host between blocks. A block
+// reads one half of the double-buffered H / q and writes the other, so the state a check accepts stays
+// intact under the block enqueued after it; accepting it restores the buffer parity, and the extra
+// block's results are never read. Iterations, residuals and the final state are those of the loop below.
+int pt_solve_pipelined(ptd_ctx* c, int64_t* iters, double* err_rms, double* err_max) {
+    const ptd_params& p = c->p;
+    const int64_t nout = p.nout;
+    const double len = p.norm_len > 0 ? p.norm_len : (double)c->nR;
+    double* slot[2] = {c->hred, c->hred + 2};
+    auto enqueue = [&](int s) -> int {
+        const int st = run_iterations(c, nout, true);
+        if (st != PTD_OK) return st;
+        PTD_HIP(hipMemcpyAsync(slot[s], c->dred, 2 * sizeof(double), hipMemcpyDeviceToHost, c->stream));
+        PTD_HIP(hipEventRecord(c->evc[s], c->stream));
+        return PTD_OK;
+    };
+    int64_t iter = 0, queued = 0;  // iterations checked / enqueued
+    double rms = 2.0 * p.tol, mx = rms, err = rms;
+    int s = 0;
+    if (nout <= p.itmax) {
+        const int st = enqueue(s);
+        if (st != PTD_OK) return st;
+        queued = nout;
+    }
+    while (queued > iter) {
+        const int hcur = c->hcur, qcur = c->qcur;  // parity of the state after the block in flight
+        const bool spec = queued + nout <= p.itmax;
+        if (spec) {
+            const int st = enqueue(1 - s);
+            if (st != PTD_OK) return st;
+            queued += nout;
+        }
+        PTD_HIP(hipEventSynchronize(c->evc[s]));
+        iter += nout;
+        rms = std::sqrt(slot[s][0]) / std::sqrt(len);
+        mx = slot[s][1];
+        err = p.stop_norm == PTD_NORM_MAX ? mx : rms;
+        if (!(err > p.tol)) {  // converged (or NaN): keep this block's state
+            c->hcur = hcur;
+            c->qcur = qcur;
+            break;
+        }
+        s = 1 - s;
+    }
+    if (err > p.tol && iter < p.itmax) {  // the iterations up to itMax, fewer than nout: no check
+        const int st = run_iterations(c, p.itmax - iter, false);
+        if (st != PTD_OK) return st;
+        iter = p.itmax;
+    }
+    PTD_HIP(hipStreamSynchronize(c->stream));
+    if (iters) *iters = iter;
+    if (err_rms) *err_rms = rms;
+    if (err_max) *err_max = mx;
+    if (std::isnan(err)) return fail(PTD_ERR_NAN, "NaN");
+    return PTD_OK;
+}
+
 // One PT solve at fixed Hold (float64 / float32 sweeps): `while err > tol && iter < itmax` with a check
 // every nout iterations (scripts/diff_3D/diff_3D_nonlin_multixpu.jl:102-115). err only changes at checks,
 // so the iterations up to the next check (or itMax) run as one block.
 int pt_solve_loop(ptd_ctx* c, int64_t* iters, double* err_rms, double* err_max) {
     const ptd_params& p = c->p;
+    const char* pipe = getenv("PTD_PIPELINE");
+    if (c->hred && c->evc[0] && c->evc[1] && ptd::use_2d_lds(c) && p.form == PTD_PLAIN &&
+        p.nout <= ptd::lds2d_max_iters() && !(pipe && pipe[0] == '0'))
+        return pt_solve_pipelined(c, iters, err_rms, err_max);
     int64_t iter = 0;
     double err = 2.0 * p.tol, rms = err, mx = err;
     while (err > p.tol && iter < p.itmax) {
@@ -525,9 +590,10 @@ int32_t ptd_ctx_create(const ptd_params* params, int32_t device, ptd_ctx** out) 
     PTD_ALLOC((void**)&c->partial, 2 * c->npartial, sizeof(double));
     PTD_ALLOC((void**)&c->dred, 2, sizeof(double));
 #undef PTD_ALLOC
-    e = hipHostMalloc((void**)&c->hred, 2 * sizeof(double), hipHostMallocDefault);
+    e = hipHostMalloc((void**)&c->hred, 4 * sizeof(double), hipHostMallocDefault);
     if (e == hipSuccess) e = hipEventCreate(&c->ev0);
     if (e == hipSuccess) e = hipEventCreate(&c->ev1);
+    for (int k = 0; k < 2 && e == hipSuccess; ++k) e = hipEventCreateWithFlags(&c->evc[k], hipEventDisableTiming);
     if (e == hipSuccess) e = hipStreamSynchronize(c->stream);
     if (e != hipSuccess) {
         free_ctx(c);

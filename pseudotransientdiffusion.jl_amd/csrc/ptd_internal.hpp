@@ -459,8 +459,9 @@ struct ptd_ctx {
     double* partial;            // per-block (sum, max) of the residual
     int64_t npartial;
     double* dred;               // device (sum, max) after the final reduction
-    double* hred;               // pinned host mirror of dred
-    hipEvent_t ev0, ev1;
+    double* hred;               // pinned host mirror of dred (two (sum, max) slots)
+    hipEvent_t ev0, ev1;        // timing (ptd_iterate_timed)
+    hipEvent_t evc[2];          // completion of the checks in flight (pipelined 2D solves)
     int64_t bytes;
     // PTD_MIXED: float32 correction sweeps around the float64 iterate H[hcur]
     bool mixed;

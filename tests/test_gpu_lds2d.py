@@ -153,3 +153,24 @@ def test_lds_checks_every_nout(script, nout, pkg, monkeypatch):
     assert all(int(v) % nout == 0 for v in out[0].iters)
     assert np.array_equal(out[0].H, out[1].H), f"rel {_relmax(out[0].H, out[1].H):.2e}"
     np.testing.assert_allclose(out[0].errs, out[1].errs, rtol=1e-12)
+
+
+@pytest.mark.parametrize("itmax", [30, 34, 100000])
+@pytest.mark.parametrize("script", ["diff_2D_lin", "diff_2D_nonlin"])
+def test_pipelined_checks_equal_blocking_loop(script, itmax, pkg, monkeypatch):
+    """Small 2D solves enqueue the next check block before reading the last residual: counts, residual
+    norms and H equal those of the loop that waits at every check, also when itMax (a multiple of nout or
+    not) ends the solve before convergence."""
+    shape = (120, 100)
+    print(f"[gpu] pipelined checks {script} itmax={itmax}", flush=True)
+    spec = pkg.SCRIPTS[script]
+    out = []
+    for pipe in ("1", "0"):
+        monkeypatch.setenv("PTD_PIPELINE", pipe)
+        H0 = pkg.initial_condition(spec, *shape)
+        out.append(pkg.run_script(script, *shape, H0=H0, itMax=itmax))
+    assert list(out[0].iters) == list(out[1].iters)
+    if itmax < 1000:
+        assert all(int(v) == itmax for v in out[0].iters)
+    assert np.array_equal(out[0].errs, out[1].errs)
+    assert np.array_equal(out[0].H, out[1].H)
