@@ -462,6 +462,8 @@ struct ptd_ctx {
     double* hred;               // pinned host mirror of dred (two (sum, max) slots)
     hipEvent_t ev0, ev1;        // timing (ptd_iterate_timed)
     hipEvent_t evc[2];          // completion of the checks in flight (pipelined 2D solves)
+    void* raw[48];              // the context's device allocations (fields may start at an offset)
+    int nraw;
     int64_t bytes;
     // PTD_MIXED: float32 correction sweeps around the float64 iterate H[hcur]
     bool mixed;
@@ -487,6 +489,8 @@ hipError_t launch_fused2(ptd_ctx* c, bool check);         // two fused PT iterat
 int64_t max_chunk_tz(const ptd_params& p, int64_t esz);
 bool fused_available(const ptd_ctx* c);
 bool fused2_available(const ptd_ctx* c);
+hipError_t launch_fused3(ptd_ctx* c);  // three fused PT iterations in one pass (3D, no residual)
+bool fused3_available(const ptd_ctx* c);
 // 2D: niter (1..lds2d_max_iters()) PT iterations in one LDS-resident launch (check: the last one's
 // plain-form residual reduced into dred), and whether a single-problem 2D solve runs its iterations that
 // way (small grids)

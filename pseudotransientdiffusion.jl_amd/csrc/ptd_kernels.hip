@@ -1201,6 +1201,272 @@ k_fused3d_t2(const KP<T> p, const T* __restrict__ Hin, T* __restrict__ Hout, con
 }
 
 // ---------------------------------------------------------------------------------------------
+// fused 3D sweep over three PT iterations (temporal blocking, three levels)
+// ---------------------------------------------------------------------------------------------
+//
+// The two-level z-marching block extended by a third level: per plane step k, level 1 (iteration n+1)
+// does plane k, level 2 (n+2) plane k-1 and level 3 (n+3) plane k-2, so HBM sees one read of the
+// iterate-n fields and one write of the iterate-(n+3) fields for three iterations. The halo shrinks by
+// one cell per level: lane L holds i = 58*bx - 2 + L and row r holds j = (R-6)*by - 2 + r; level 1 updates
+// lanes 1..62 / rows 1..R-2, level 2 lanes 2..61 / rows 2..R-3, level 3 lanes 3..60 / rows 3..R-4. Level 1
+// runs the planes k0-2 .. k1+1 around the chunk, level 2 k0-1 .. k1, level 3 updates k0 .. k1-1. Cells on the
+// domain boundary keep H at every level. Each level evaluates exactly the expressions of the one-iteration
+// sweep, so a pass gives the same bits as three k_fused3d launches. (Plain / perf forms of the lin, step
+// and nonlinear D cases; no residual: the checking iteration of a solve stays a two-level or single sweep.)
+template <typename T, int DC, int FORM, int ARITH, int R>
+__device__ __forceinline__ void fused3d_t3_tile(const KP<T>& p, const T* __restrict__ Hin, T* __restrict__ Hout,
+                                                const T* __restrict__ Hold, const T* __restrict__ D,
+                                                const T* __restrict__ qxi, const T* __restrict__ qyi,
+                                                const T* __restrict__ qzi, T* __restrict__ qxo,
+                                                T* __restrict__ qyo, T* __restrict__ qzo, int bx, int by, int bz,
+                                                int tz) {
+    static_assert(DC == PTD_LIN || DC == PTD_STEP || DC == PTD_NONLIN, "three-level sweep: lin / step / nonlin");
+    using Ph = Phys<T, DC, FORM, ARITH>;
+    constexpr bool STEP = DC == PTD_STEP;
+    __shared__ T sH[R][64];    // H at plane k (level-1 y neighbours)
+    __shared__ T sH1[R][64];   // H1 at plane k-1 (level-2 y neighbours)
+    __shared__ T sH2[R][64];   // H2 at plane k-2 (level-3 y neighbours)
+    __shared__ T sQ1[R][64];   // level-1 y flux at plane k
+    __shared__ T sQ2[R][64];   // level-2 y flux at plane k-1
+    __shared__ T sQ3[R][64];   // level-3 y flux at plane k-2
+    __shared__ T sD0[STEP ? R : 1][64];  // D at planes k, k-1, k-2
+    __shared__ T sD1[STEP ? R : 1][64];
+    __shared__ T sD2[STEP ? R : 1][64];
+
+    const int lane = threadIdx.x, row = threadIdx.y;
+    const int nx = (int)p.nx, ny = (int)p.ny, nz = (int)p.nz;
+    const int wrow = __builtin_amdgcn_readfirstlane(row);
+    // wave-uniform row roles: level L updates rows L..R-1-L and evaluates lower y faces on rows L..R-L
+    const bool l1all = wrow >= 1 && wrow <= R - 2, l1y = wrow >= 1;
+    const bool l2all = wrow >= 2 && wrow <= R - 3, l2y = wrow >= 2 && wrow <= R - 2;
+    const bool l3all = wrow >= 3 && wrow <= R - 4, l3y = wrow >= 3 && wrow <= R - 3;
+    const int i = 58 * bx - 2 + lane;
+    const int j = (R - 6) * by - 2 + row;
+    const int k0 = 1 + tz * bz;
+    const int k1 = (k0 + tz < nz - 1) ? k0 + tz : nz - 1;  // level 3 updates planes k0..k1-1
+    const int ka = k0 - 2 > 0 ? k0 - 2 : 0;                 // level 1 runs planes ka..k1+1
+    const int kb = ka > 0 ? ka - 1 : 0;
+
+    const bool xin = (i >= 1) && (i <= nx - 2);
+    const bool yin = (j >= 1) && (j <= ny - 2);
+    const bool u1 = (lane >= 1) && (lane <= 62) && (row >= 1) && (row <= R - 2) && xin && yin;
+    const bool u2 = (lane >= 2) && (lane <= 61) && (row >= 2) && (row <= R - 3) && xin && yin;
+    const bool r3mid = (row >= 3) && (row <= R - 4);
+    const bool l3mid = (lane >= 3) && (lane <= 60);
+    const bool u3 = l3mid && r3mid && xin && yin;
+    const bool f3x = (lane >= 3) && (lane <= 61) && (i <= nx - 1) && r3mid && yin;
+    const bool f3y = (row >= 3) && (row <= R - 3) && (j <= ny - 1) && l3mid && xin;
+    const bool w3x = f3x && ((lane <= 60 && i <= nx - 2) || i == nx - 1);
+    const bool w3y = f3y && ((row <= R - 4 && j <= ny - 2) || j == ny - 1);
+
+    // clamped byte offsets (every load in bounds); Hold, qx, qz load on rows 1..R-2 and qy on rows 1..R-1
+    // (the rows level 1 uses them on), other rows repeat the nearest such row's address
+    const size_t psH = (size_t)nx * ny;
+    const size_t psQx = (size_t)(nx - 1) * (ny - 2);
+    const size_t psQy = (size_t)(nx - 2) * (ny - 1);
+    const size_t psQz = (size_t)(nx - 2) * (ny - 2);
+    const int jr = (R - 6) * by - 2 + max(1, min(row, R - 2));
+    const int jq = (R - 6) * by - 2 + max(1, row);
+    const int ic = max(0, min(i, nx - 1)), jc = max(0, min(j, ny - 1)), jo = max(0, min(jr, ny - 1));
+    const int ixq = max(1, min(i, nx - 1)), jxq = max(1, min(jr, ny - 2));
+    const int iyq = max(1, min(i, nx - 2)), jyq = max(1, min(jq, ny - 1));
+    const int jzq = max(1, min(jr, ny - 2));
+    constexpr uint32_t es = (uint32_t)sizeof(T);
+    const uint32_t oH = (uint32_t)(ic + nx * jc) * es;
+    const uint32_t oHo = (uint32_t)(ic + nx * jo) * es;
+    const uint32_t oQx = (uint32_t)((ixq - 1) + (nx - 1) * (jxq - 1)) * es;
+    const uint32_t oQy = (uint32_t)((iyq - 1) + (nx - 2) * (jyq - 1)) * es;
+    const uint32_t oQz = (uint32_t)((iyq - 1) + (nx - 2) * (jzq - 1)) * es;
+
+    // every plane the tile touches lies in kb .. kb + tz + 5: block-uniform plane-kb bases, 32-bit offsets
+    const char* const bHin = reinterpret_cast<const char*>(Hin + psH * kb);
+    const char* const bD = reinterpret_cast<const char*>(D + (STEP ? psH * kb : 0));
+    const char* const bHo = reinterpret_cast<const char*>(Hold + psH * kb);
+    const char* const bqx = reinterpret_cast<const char*>(qxi + psQx * kb);
+    const char* const bqy = reinterpret_cast<const char*>(qyi + psQy * kb);
+    const char* const bqz = reinterpret_cast<const char*>(qzi + psQz * kb);
+    char* const bHout = reinterpret_cast<char*>(Hout + psH * kb);
+    char* const bqxo = reinterpret_cast<char*>(qxo + psQx * kb);
+    char* const bqyo = reinterpret_cast<char*>(qyo + psQy * kb);
+    char* const bqzo = reinterpret_cast<char*>(qzo + psQz * kb);
+    const uint32_t pbH = (uint32_t)psH * es;
+    const uint32_t pbQx = (uint32_t)psQx * es, pbQy = (uint32_t)psQy * es, pbQz = (uint32_t)psQz * es;
+
+    auto load_plane = [&](int pl) {  // level-1 operands of plane pl, pl in [0, nz-1]
+        PlaneOps2<T, T> o;
+        const uint32_t ph = (uint32_t)((pl + 1 < nz ? pl + 1 : nz - 1) - kb);
+        const uint32_t pq = (uint32_t)((pl - 1 < 0 ? 0 : (pl - 1 > nz - 3 ? nz - 3 : pl - 1)) - kb);
+        const uint32_t pz = (uint32_t)((pl > nz - 2 ? nz - 2 : pl) - kb);
+        o.Hn = ldc<T>(bHin, oH + ph * pbH);
+        if constexpr (STEP) o.Dn = ldc<T>(bD, oH + ph * pbH);
+        else o.Dn = T{};
+        o.Ho = ldc<T>(bHo, oHo + (uint32_t)(pl - kb) * pbH);
+        o.qx = ldc<T>(bqx, oQx + pq * pbQx);
+        o.qy = ldc<T>(bqy, oQy + pq * pbQy);
+        o.qz = ldc<T>(bqz, oQz + pz * pbQz);
+        return o;
+    };
+
+    // level-1 lower z face of plane ka (face (ka-1 | ka)); needed only when plane ka is inner
+    T Hc = ldc<T>(bHin, oH + (uint32_t)(ka - kb) * pbH);
+    T Dc{}, Dm1{}, Dm2{}, Dm3{};  // D at planes k, k-1, k-2, k-3
+    if constexpr (STEP) {
+        Dc = ldc<T>(bD, oH + (uint32_t)(ka - kb) * pbH);
+        Dm1 = ldc<T>(bD, oH);
+    }
+    T unused = 0;
+    T q1zL = Ph::template face<false>(p, ldc<T>(bqz, oQz), ldc<T>(bHin, oH), Hc, Dm1, Dc, p.dz, p._dz, p._dz8, unused);
+    PlaneOps2<T, T> A = load_plane(ka);
+
+    T H1p = 0, H2p = 0;          // H1 at plane k-1, H2 at plane k-2
+    T Hop = 0, Hopp = 0;         // Hold at planes k-1, k-2
+    T q1xp = 0, q1yp = 0;        // level-1 fluxes on the lower x / y faces at plane k-1
+    T q2xp = 0, q2yp = 0;        // level-2 fluxes on the lower x / y faces at plane k-2
+    T q2zL = 0, q3zL = 0;        // level-2 flux on face (k-2 | k-1), level-3 flux on face (k-3 | k-2)
+
+    for (int k = ka; k <= k1 + 1; ++k) {
+        // phase A: planes into LDS
+        sH[row][lane] = Hc;
+        sH1[row][lane] = H1p;
+        sH2[row][lane] = H2p;
+        if constexpr (STEP) {
+            sD0[row][lane] = Dc;
+            sD1[row][lane] = Dm1;
+            sD2[row][lane] = Dm2;
+        }
+        __syncthreads();
+
+        // phase B: x / y faces of level 1 (plane k), level 2 (plane k-1), level 3 (plane k-2)
+        const int rs = row > 0 ? row - 1 : 0, rn = row < R - 1 ? row + 1 : R - 1;
+        T Dw0{}, De0{}, Ds0{}, Dn0{}, Dw1{}, De1{}, Ds1{}, Dn1{}, Dw2{}, De2{}, Ds2{}, Dn2{};
+        if constexpr (STEP) {
+            Dw0 = shfl_up1(Dc);
+            De0 = shfl_down1(Dc);
+            Ds0 = sD0[rs][lane];
+            Dn0 = sD0[rn][lane];
+            Dw1 = shfl_up1(Dm1);
+            De1 = shfl_down1(Dm1);
+            Ds1 = sD1[rs][lane];
+            Dn1 = sD1[rn][lane];
+            Dw2 = shfl_up1(Dm2);
+            De2 = shfl_down1(Dm2);
+            Ds2 = sD2[rs][lane];
+            Dn2 = sD2[rn][lane];
+        }
+        const T Hw = __shfl_up(Hc, 1);
+        const T Hs = sH[rs][lane];
+        T q1x = 0, q1y = 0, q1zU = 0;
+        if (l1y) q1y = Ph::template face<false>(p, A.qy, Hs, Hc, Ds0, Dc, p.dy, p._dy, p._dy8, unused);
+        if (l1all) {
+            q1x = Ph::template face<false>(p, A.qx, Hw, Hc, Dw0, Dc, p.dx, p._dx, p._dx8, unused);
+            q1zU = Ph::template face<false>(p, A.qz, Hc, A.Hn, Dc, A.Dn, p.dz, p._dz, p._dz8, unused);
+        }
+        const T H1w = __shfl_up(H1p, 1);
+        const T H1s = sH1[rs][lane];
+        T q2x = 0, q2y = 0;
+        if (l2y) q2y = Ph::template face<false>(p, q1yp, H1s, H1p, Ds1, Dm1, p.dy, p._dy, p._dy8, unused);
+        if (l2all) q2x = Ph::template face<false>(p, q1xp, H1w, H1p, Dw1, Dm1, p.dx, p._dx, p._dx8, unused);
+        const T H2w = __shfl_up(H2p, 1);
+        const T H2s = sH2[rs][lane];
+        T q3x = 0, q3y = 0;
+        if (l3y) q3y = Ph::template face<false>(p, q2yp, H2s, H2p, Ds2, Dm2, p.dy, p._dy, p._dy8, unused);
+        if (l3all) q3x = Ph::template face<false>(p, q2xp, H2w, H2p, Dw2, Dm2, p.dx, p._dx, p._dx8, unused);
+        sQ1[row][lane] = q1y;
+        sQ2[row][lane] = q2y;
+        sQ3[row][lane] = q3y;
+        __syncthreads();
+
+        // phase C: level-1 update of plane k; level-2 z face (k-1 | k) and update of plane k-1; level-3 z
+        // face (k-2 | k-1) and update of plane k-2
+        T dmax1 = 0, dmax2 = 0, dmax3 = 0;
+        if constexpr (STEP) {
+            dmax1 = fmax(fmax(Dw0, De0), Dc);
+            dmax1 = fmax(dmax1, fmax(Ds0, Dn0));
+            dmax1 = fmax(dmax1, fmax(Dm1, A.Dn));
+            dmax2 = fmax(fmax(Dw1, De1), Dm1);
+            dmax2 = fmax(dmax2, fmax(Ds1, Dn1));
+            dmax2 = fmax(dmax2, fmax(Dm2, Dc));
+            dmax3 = fmax(fmax(Dw2, De2), Dm2);
+            dmax3 = fmax(dmax3, fmax(Ds2, Dn2));
+            dmax3 = fmax(dmax3, fmax(Dm3, Dm1));
+        }
+        T H1 = Hc;  // boundary cells, boundary planes and rows without a level-1 update keep H
+        if (l1all) {
+            const T q1xE = __shfl_down(q1x, 1);
+            const T q1yN = sQ1[rn][lane];
+            const T dv1 = Ph::template div<3>(p, q1xE - q1x, q1yN - q1y, q1zU - q1zL);
+            const T H1c = Ph::cell(p, Hc, A.Ho, dv1, dmax1, Dc);
+            if (u1 && k >= 1 && k <= nz - 2) H1 = H1c;
+        }
+        T q2zU = 0, H2 = H1p;
+        if (l2all) {
+            q2zU = Ph::template face<false>(p, q1zL, H1p, H1, Dm1, Dc, p.dz, p._dz, p._dz8, unused);
+            const T q2xE = __shfl_down(q2x, 1);
+            const T q2yN = sQ2[rn][lane];
+            const T dv2 = Ph::template div<3>(p, q2xE - q2x, q2yN - q2y, q2zU - q2zL);
+            const T H2c = Ph::cell(p, H1p, Hop, dv2, dmax2, Dm1);
+            if (u2 && k - 1 >= 1 && k - 1 <= nz - 2) H2 = H2c;
+        }
+        if (l3all) {
+            const T q3zU = Ph::template face<false>(p, q2zL, H2p, H2, Dm2, Dm1, p.dz, p._dz, p._dz8, unused);
+            const int m = k - 2;
+            if (m >= k0 && m <= k1 - 1) {
+                const T q3xE = __shfl_down(q3x, 1);
+                const T q3yN = sQ3[rn][lane];
+                const T dv3 = Ph::template div<3>(p, q3xE - q3x, q3yN - q3y, q3zU - q3zL);
+                const T H3 = Ph::cell(p, H2p, Hopp, dv3, dmax3, Dm2);
+                const uint32_t rm = (uint32_t)(m - kb);
+                if (u3) {
+                    stc<T>(bHout, oH + rm * pbH, H3);
+                    stc<T>(bqzo, oQz + rm * pbQz, q3zU);
+                }
+                if (w3x) stc<T>(bqxo, oQx + (rm - 1) * pbQx, q3x);
+            } else if (m == 0 && k0 == 1) {
+                if (u3) stc<T>(bqzo, oQz, q3zU);  // face (0 | 1): lower face of the first inner plane (kb = 0)
+            }
+            q3zL = q3zU;
+        }
+        if (w3y && k - 2 >= k0 && k - 2 <= k1 - 1) stc<T>(bqyo, oQy + (uint32_t)(k - 3 - kb) * pbQy, q3y);
+        // carry to plane k+1
+        q2zL = q2zU;
+        q1zL = q1zU;
+        H2p = H2;
+        H1p = H1;
+        Hopp = Hop;
+        Hop = A.Ho;
+        q2xp = q2x;
+        q2yp = q2y;
+        q1xp = q1x;
+        q1yp = q1y;
+        if constexpr (STEP) {
+            Dm3 = Dm2;
+            Dm2 = Dm1;
+            Dm1 = Dc;
+            Dc = A.Dn;
+        }
+        Hc = A.Hn;
+        A = load_plane(k + 1 <= nz - 1 ? k + 1 : nz - 1);
+    }
+}
+
+static constexpr int kRows3 = 16;  // rows (waves) per block of the three-level sweep
+
+template <typename T, int DC, int FORM, int ARITH, int R>
+__global__ void __launch_bounds__(64 * R)
+k_fused3d_t3(const KP<T> p, const T* __restrict__ Hin, T* __restrict__ Hout, const T* __restrict__ Hold,
+             const T* __restrict__ D, const T* __restrict__ qxi, const T* __restrict__ qyi,
+             const T* __restrict__ qzi, T* __restrict__ qxo, T* __restrict__ qyo, T* __restrict__ qzo, int tz,
+             int ntx, int nty, int ntiles, int per_xcd) {
+    const int b = blockIdx.x;
+    const int tile = (b & 7) * per_xcd + (b >> 3);  // XCD-aware: each XCD walks a contiguous tile range
+    if (tile >= ntiles) return;
+    const int bx = tile % ntx;
+    const int rest = tile / ntx;
+    fused3d_t3_tile<T, DC, FORM, ARITH, R>(p, Hin, Hout, Hold, D, qxi, qyi, qzi, qxo, qyo, qzo, bx, rest % nty,
+                                           rest / nty, tz);
+}
+
+// ---------------------------------------------------------------------------------------------
 // launchers
 // ---------------------------------------------------------------------------------------------
 
@@ -1493,6 +1759,25 @@ static hipError_t sweep2d_lds_run(ptd_ctx* c, const SweepBufs& b, int niter, boo
     return hipGetLastError();
 }
 
+// One three-level pass (three iterations, no residual).
+template <typename T, int DC, int FORM, int ARITH>
+static hipError_t sweep3_run(ptd_ctx* c, const SweepBufs& b) {
+    const KP<T> kp = make_kp<T>(c->p);
+    // a chunk touches tz + 6 planes from its base (32-bit offsets): one plane fewer than the two-level limit
+    const int tz = (int)std::max<int64_t>(1, std::min<int64_t>(fused_tz(c, false), max_chunk_tz(c->p, (int64_t)c->esz) - 1));
+    constexpr int R = kRows3;
+    const int ntx = (int)((c->p.nx - 2 + 57) / 58);
+    const int nty = (int)((c->p.ny - 2 + R - 7) / (R - 6));
+    const int nbz = (int)((c->p.nz - 2 + tz - 1) / tz);
+    const int ntiles = ntx * nty * nbz;
+    const int per = (ntiles + 7) / 8;
+    hipLaunchKernelGGL((k_fused3d_t3<T, DC, FORM, ARITH, R>), dim3(8 * per), dim3(64, R), 0, c->stream, kp,
+                       (const T*)b.Hin, (T*)b.Hout, (const T*)b.Hold, (const T*)b.aux, (const T*)b.qin[0],
+                       (const T*)b.qin[1], (const T*)b.qin[2], (T*)b.qout[0], (T*)b.qout[1], (T*)b.qout[2], tz, ntx,
+                       nty, ntiles, per);
+    return hipGetLastError();
+}
+
 // The context's own fields (H, q double-buffered; the sweep flips them).
 static SweepBufs ctx_bufs(ptd_ctx* c) {
     const int h = c->hcur, q = c->qcur;
@@ -1531,6 +1816,16 @@ template <typename T, int DC, int FORM>
 static hipError_t fused_t(ptd_ctx* c, bool check) { return ctx_sweep_t<T, DC, FORM, false>(c, check); }
 template <typename T, int DC, int FORM>
 static hipError_t fused2_t(ptd_ctx* c, bool check) { return ctx_sweep_t<T, DC, FORM, true>(c, check); }
+template <typename T, int DC, int FORM>
+static hipError_t fused3_t(ptd_ctx* c) {
+    const SweepBufs b = ctx_bufs(c);
+    const hipError_t e = c->p.arith == PTD_ARITH_FAST ? sweep3_run<T, DC, FORM, kFast>(c, b)
+                                                      : sweep3_run<T, DC, FORM, kExact>(c, b);
+    if (e != hipSuccess) return e;
+    c->hcur = 1 - c->hcur;
+    c->qcur = 1 - c->qcur;
+    return hipSuccess;
+}
 template <typename T, int DC, int FORM>
 static hipError_t lds2d_t(ptd_ctx* c, int niter, bool check) {
     const SweepBufs b = ctx_bufs(c);
@@ -1571,6 +1866,23 @@ hipError_t launch_lds2d(ptd_ctx* c, int niter, bool check) {
     if (niter < 1 || niter > kLdsHalo || c->p.ndim != 2 || (check && c->p.form != PTD_PLAIN))
         return hipErrorInvalidValue;
     PTD_DISPATCH(lds2d_t, c, niter, check);
+}
+hipError_t launch_fused3(ptd_ctx* c) {
+    if (c->p.ndim != 3) return hipErrorInvalidValue;
+    PTD_DISPATCH(fused3_t, c);
+}
+// Three-level passes (3D grids whose chunk keeps its planes under 32-bit offsets) pay where the two-level
+// sweep is bound by its bytes with VALU to spare: the float64 FAST linear and step sweeps. Measured at
+// 511^3, ms per iteration two- / three-level: FAST lin 1.212 / 1.015, step 1.315 / 1.246, nonlinear
+// 1.235 / 1.380; EXACT lin 1.44 / 1.66, step 2.021 / 2.019, nonlinear 2.21 / 2.42; float32 nonlinear
+// 0.653 / 0.907, lin 0.615 / 0.731 (the third level's halo work and registers cost more than its bytes
+// save). PTD_T3=0 / 1 in the environment switches them off / on for every 3D grid.
+bool fused3_available(const ptd_ctx* c) {
+    if (!(c->p.ndim == 3 && fused_available(c) && max_chunk_tz(c->p, (int64_t)c->esz) >= 2)) return false;
+    const char* env = getenv("PTD_T3");
+    if (env && env[0] == '0') return false;
+    if (env && env[0] == '1') return true;
+    return c->esz == 8 && c->p.arith == PTD_ARITH_FAST && (c->p.dcase == PTD_LIN || c->p.dcase == PTD_STEP);
 }
 int lds2d_max_iters() { return kLdsHalo; }
 

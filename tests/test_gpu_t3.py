@@ -1,0 +1,115 @@
+"""GPU parity of the three-level 3D sweep (k_fused3d_t3: three PT iterations per HBM pass), against the CPU
+oracle and against the one- and two-level sweeps.
+
+ - EXACT arithmetic: H and the three fluxes bit-identical to the oracle's niter iterations (3 = one pass,
+   7 = two passes and one single sweep, 8 = two passes and one two-level pass) for the four 3D scripts
+   (scripts/diff_3D/*.jl compute_flux! + compute_update!), on ragged grids, the smallest grid and grids
+   whose z chunks (4 planes, and the default depth) split the columns;
+ - FAST arithmetic: the same bits as the same iterations through the two-level / single sweeps;
+ - a converging solve with checks (every nout = 10: three-level passes, then a checking two-level pass):
+   the oracle's counts and H.
+"""
+import numpy as np
+import pytest
+
+pytestmark = pytest.mark.gpu
+
+SCRIPTS_3D = ["diff_3D_lin_multixpu", "diff_3D_linstep_multixpu", "diff_3D_nonlin_multixpu", "diff_3D_nonlin_perf"]
+SHAPES = [(37, 29, 21), (70, 33, 18), (131, 19, 47), (3, 3, 3), (64, 16, 5), (61, 62, 40)]
+
+
+def _relmax(a, b):
+    return float(np.max(np.abs(a - b)) / max(np.max(np.abs(b)), 1e-300))
+
+
+@pytest.fixture(scope="module", autouse=True)
+def _device(pkg):
+    lib = pkg.load()
+    if lib.ptd_device_check(0, None, 0) != 0:
+        pytest.fail(f"no gfx950 device: {lib.ptd_last_error()}")
+
+
+def _random_state(pkg, orc, script, shape, seed, arith=0, precision=0):
+    rng = np.random.default_rng(seed)
+    spec = pkg.SCRIPTS[script]
+    params = pkg.make_params(spec, *shape, arith=arith, precision=precision)
+    case = orc.make_case(orc.SCRIPTS[script], *shape)
+    H = np.asfortranarray(rng.uniform(1.0, 1.2, size=shape))
+    Hold = np.asfortranarray(rng.uniform(1.0, 1.2, size=shape))
+    D = np.asfortranarray(np.where(rng.uniform(size=shape) < 0.5, 1e-4, 1.0)) if spec["dcase"] == pkg.STEP else None
+    q = {name: np.asfortranarray(0.05 * rng.normal(size=pkg.field_shape(params, fid)))
+         for name, fid in (("qHx", 3), ("qHy", 4), ("qHz", 5))}
+    return params, case, H, Hold, D, q
+
+
+def _gpu_iterate(pkg, params, H, Hold, D, q, niter):
+    with pkg.Context(params) as ctx:
+        ctx.set("H", H)
+        ctx.set("Hold", Hold)
+        for k, v in q.items():
+            ctx.set(k, v)
+        if D is not None:
+            ctx.set("D", D)
+        ctx.iterate(niter)
+        return ctx.get("H"), {k: ctx.get(k) for k in q}
+
+
+def _oracle_iterate(orc, pkg, params, case, H, Hold, D, q, niter):
+    dtau = None
+    if D is not None:
+        dtau = np.zeros(pkg.field_shape(params, 10), order="F")
+        orc.lib().ptdo_iter_params(orc.ctypes.byref(case), orc._p(D), orc._p(dtau))
+    Hr = H.copy(order="F")
+    qr = {k: v.copy(order="F") for k, v in q.items()}
+    orc.lib().ptdo_iterate(orc.ctypes.byref(case), orc._p(Hr), orc._p(Hold), orc._p(D), orc._p(dtau),
+                           orc._p(qr["qHx"]), orc._p(qr["qHy"]), orc._p(qr["qHz"]), niter)
+    return Hr, qr
+
+
+@pytest.mark.parametrize("tz", ["4", "default"])
+@pytest.mark.parametrize("niter", [3, 7, 8])
+@pytest.mark.parametrize("shape", SHAPES)
+@pytest.mark.parametrize("script", SCRIPTS_3D)
+def test_t3_sweeps_match_oracle(script, shape, niter, tz, pkg, orc, monkeypatch):
+    print(f"[gpu] t3 {script} {shape} niter={niter} tz={tz}", flush=True)
+    monkeypatch.setenv("PTD_T3", "1")
+    if tz == "default":
+        monkeypatch.delenv("PTD_TZ", raising=False)
+    else:
+        monkeypatch.setenv("PTD_TZ", tz)
+    params, case, H, Hold, D, q = _random_state(pkg, orc, script, shape, seed=sum(shape) + niter)
+    Hg, qg = _gpu_iterate(pkg, params, H, Hold, D, q, niter)
+    Hr, qr = _oracle_iterate(orc, pkg, params, case, H, Hold, D, q, niter)
+    assert np.all(np.isfinite(Hr))
+    assert np.array_equal(Hg, Hr), f"H: rel {_relmax(Hg, Hr):.2e}"
+    for k in q:
+        assert np.array_equal(qg[k], qr[k]), f"{k}: rel {_relmax(qg[k], qr[k]):.2e}"
+
+
+@pytest.mark.parametrize("precision", [0, 1], ids=["f64", "f32"])
+@pytest.mark.parametrize("script", SCRIPTS_3D)
+def test_t3_fast_equals_two_level(script, precision, pkg, orc, monkeypatch):
+    shape, niter = (70, 33, 18), 9
+    print(f"[gpu] t3 fast vs two-level {script} precision={precision}", flush=True)
+    monkeypatch.setenv("PTD_TZ", "4")
+    params, case, H, Hold, D, q = _random_state(pkg, orc, script, shape, seed=3, arith=1, precision=precision)
+    out = []
+    for t3 in ("1", "0"):
+        monkeypatch.setenv("PTD_T3", t3)
+        out.append(_gpu_iterate(pkg, params, H, Hold, D, q, niter))
+    assert np.all(np.isfinite(out[1][0]))
+    assert np.array_equal(out[0][0], out[1][0]), f"H: rel {_relmax(out[0][0], out[1][0]):.2e}"
+    for k in q:
+        assert np.array_equal(out[0][1][k], out[1][1][k]), k
+
+
+@pytest.mark.parametrize("script", ["diff_3D_nonlin_multixpu", "diff_3D_linstep_multixpu", "diff_3D_lin_multixpu"])
+def test_t3_solve_with_checks_matches_oracle(script, pkg, orc, monkeypatch):
+    shape = (45, 38, 29)
+    print(f"[gpu] t3 solve {script} {shape}", flush=True)
+    monkeypatch.setenv("PTD_T3", "1")
+    ref = orc.run_script(script, *shape)
+    res = pkg.run_script(script, *shape, H0=ref["H0"], D=ref["D"])
+    assert list(res.iters) == list(ref["iters"])
+    assert np.array_equal(res.H, ref["H"]), f"rel {_relmax(res.H, ref['H']):.2e}"
+    np.testing.assert_allclose(res.errs, ref["errs"], rtol=1e-12)
