@@ -235,7 +235,7 @@ int run_iterations(ptd_ctx* c, int64_t n, bool check_last) {
             const int m = (int)std::min<int64_t>(n - (check_last ? 1 : 0), lmax);
             PTD_HIP(ptd::launch_lds2d(c, m, false));
             n -= m;
-        } else if (t3 && n - (check_last ? 2 : 0) >= 3) {  // the checking iteration ends a two-level pass
+        } else if (ptd::next_launch_iters(n, check_last, t2, t3) == 3) {  // checks end a two-level pass
             PTD_HIP(ptd::launch_fused3(c));
             n -= 3;
         } else if (t2 && n >= 2) {

@@ -55,6 +55,9 @@ struct Group {
     int n_nc[kNumCandidates] = {};
     BatchTile* d_t2 = nullptr;  // single-level 2D groups: two-level tiles of their pair launches
     int n_t2 = 0;
+    BatchTile* d_t3[kNumCandidates] = {};  // three-level groups: tiles of their three-level passes
+    int n_t3[kNumCandidates] = {};
+    std::vector<int> plan;      // PT iterations of each launch of a check block (block_plan)
     int sub = 1;                // check blocks per replay
     uint64_t* d_span = nullptr; // (start, sum) of the group's device time per replay, summed over replays
     int32_t* d_active = nullptr; // members of the group still running, after each replay
@@ -87,12 +90,29 @@ int group_sub_blocks(const std::vector<ptd_params>& params, const std::vector<in
     return (int)std::min<int64_t>(kMaxSubBlocks, std::max<int64_t>(1, ng / n3));
 }
 
-// Launches per check block of nout iterations: two-level sweeps take two iterations per launch. The
-// single-level 2D groups (grids too small for ptd_solve to pick the two-level 2D sweep) still run
-// their first nout - 2 iterations as two-level pairs, which give the same bits; only the checking
-// launch must use the single-problem tiling (its residual partials decide when a member stops), and it
-// follows one single-level launch.
-int launches_per_block(int kind, int nout) { return kind == ptd::kSweep2D ? nout / 2 + 1 : nout / 2; }
+// PT iterations of each launch of one check block of nout iterations. Only the checking launch must
+// use the single-problem tiling (its residual partials decide when a member stops); every other launch
+// gives the same bits whatever its depth or tiling. 3D members follow the single-problem rule
+// (ptd::next_launch_iters: three-level passes for the float64 FAST linear / step members, two-level
+// pairs, a checking two-level pass last), the two-level 2D groups run pairs, and the single-level 2D
+// groups (grids too small for ptd_solve to pick the two-level 2D sweep) run their first nout - 2
+// iterations as two-level pairs, then one single-level sweep and the single-problem checking launch.
+std::vector<int> block_plan(const ptd_params& p) {
+    std::vector<int> plan;
+    if (ptd::batch_kind(p) == ptd::kSweep2D) {
+        for (int i = 0; i < p.nout / 2 - 1; ++i) plan.push_back(2);
+        plan.push_back(1);
+        plan.push_back(1);
+        return plan;
+    }
+    const bool t3 = ptd::batch_t3(p);
+    for (int64_t n = p.nout; n > 0;) {
+        const int k = ptd::next_launch_iters(n, true, true, t3);
+        plan.push_back(k);
+        n -= k;
+    }
+    return plan;
+}
 
 // Estimated work of a member: cells times PT iterations, which grow with the grid's extent (the reference's
 // published counts: about 6 n iterations over 5 steps for linear and step D, 2 n for D = H^3).
@@ -203,6 +223,13 @@ void free_batch(ptd_batch* b) {
     delete b;
 }
 
+// tiles of one member's three-level pass with z chunk depth tz
+int member_tiles3(const ptd_params& p, int tz) {
+    const int r = ptd::fused3_rows();
+    const int64_t ntx = (p.nx - 2 + 57) / 58, nty = (p.ny - 2 + r - 7) / (r - 6), nbz = (p.nz - 2 + tz - 1) / tz;
+    return (int)(ntx * nty * nbz);
+}
+
 // tiles of one member's launch: rows per block (two-level 2D: rows per tile chunk), z chunk depth tz (3D)
 int member_tiles(const ptd_params& p, int kind, int rows, int tz) {
     if (kind == ptd::kSweep2D2) return (int)(((p.nx - 2 + 59) / 60) * ((p.ny - 2 + rows - 1) / rows));
@@ -232,11 +259,15 @@ int enqueue_group(ptd_batch* b, size_t gi, int cand) {
     Group& g = b->groups[gi];
     const int nm = (int)g.members.size();
     PTD_HIPB(ptd::launch_batch_span(g.stream, g.d_span, false));
-    const int lpb = launches_per_block(g.kind, b->nout);
+    const int lpb = (int)g.plan.size();
     for (int sb = 0; sb < g.sub; ++sb) {
         for (int li = 0; li < lpb; ++li) {
             const bool check = li == lpb - 1;
-            if (g.kind == ptd::kSweep2D && li < lpb - 2) {  // a two-level pair
+            if (g.plan[(size_t)li] == 3) {  // a three-level pass
+                PTD_HIPB(ptd::launch_batch_sweep3(g.stream, g.dcase, g.arith, b->dm, g.d_t3[cand], g.n_t3[cand], li));
+                continue;
+            }
+            if (g.kind == ptd::kSweep2D && g.plan[(size_t)li] == 2) {  // a two-level pair
                 PTD_HIPB(ptd::launch_batch_sweep(g.stream, ptd::kSweep2D2, g.dcase, g.arith,
                                                  ptd::batch_rows_nocheck(ptd::kSweep2D2, g.dcase, g.arith, 0),
                                                  false, b->dm, g.d_t2, g.n_t2, li));
@@ -371,7 +402,7 @@ int32_t ptd_batch_create(const ptd_params* params, int32_t n, int32_t device, pt
         M.dcase = p.dcase;
         M.arith = p.arith;
         M.nout = p.nout;
-        M.lpb = launches_per_block(ptd::batch_kind(p), p.nout);
+        M.lpb = (int32_t)block_plan(p).size();
         M.stop_norm = p.stop_norm;
         M.tol = p.tol;
         M.norm_len = p.norm_len > 0 ? p.norm_len : (double)(mx * my * mz);
@@ -388,6 +419,7 @@ int32_t ptd_batch_create(const ptd_params* params, int32_t n, int32_t device, pt
             g->kind = ptd::batch_kind(p);
             g->dcase = p.dcase;
             g->arith = p.arith;
+            g->plan = block_plan(p);  // a function of (kind, dcase, arith) and the common nout
         }
         g->members.push_back(m);
     }
@@ -447,6 +479,19 @@ int32_t ptd_batch_create(const ptd_params* params, int32_t n, int32_t device, pt
             }
             g.n_nc[c] = (int)t.size();
             st = upload_table(b, t, &g.d_nc[c]);
+            if (st != PTD_OK) return fail_free(st);
+            if (std::find(g.plan.begin(), g.plan.end(), 3) == g.plan.end()) continue;
+            t.clear();
+            for (int32_t m : g.members) {
+                const ptd_params& p = params[m];
+                // a three-level chunk touches tz + 6 planes from its base (32-bit offsets)
+                const int tz = (int)std::min<int64_t>(std::min<int64_t>(kCandidates[c].tz3d, p.nz - 2),
+                                                      ptd::max_chunk_tz(p, 8) - 1);
+                const int nt = member_tiles3(p, tz);
+                for (int k = 0; k < nt; ++k) t.push_back({m, k, tz, 0});
+            }
+            g.n_t3[c] = (int)t.size();
+            st = upload_table(b, t, &g.d_t3[c]);
             if (st != PTD_OK) return fail_free(st);
         }
         if (g.kind == ptd::kSweep2D) {

@@ -491,6 +491,15 @@ bool fused_available(const ptd_ctx* c);
 bool fused2_available(const ptd_ctx* c);
 hipError_t launch_fused3(ptd_ctx* c);  // three fused PT iterations in one pass (3D, no residual)
 bool fused3_available(const ptd_ctx* c);
+// whether the 3D sweeps of a problem with these parameters and element size take three-level passes
+// (the float64 FAST linear and step sweeps; PTD_T3=0 / 1 in the environment overrides)
+bool fused3_wanted(const ptd_params& p, int64_t esz);
+// PT iterations of the next launch of a 3D run of n iterations (3: a three-level pass, 2: a two-level
+// pass, 1: a single sweep), given which passes are available. With check_last the run ends on a checking
+// two-level pass whenever n >= 2: three-level passes are taken while they leave at least two iterations
+// and never leave exactly three (which would end on a single checking sweep). Single-problem solves and
+// batches both follow it, so their checking launches have the same tiling.
+int next_launch_iters(int64_t n, bool check_last, bool t2, bool t3);
 // 2D: niter (1..lds2d_max_iters()) PT iterations in one LDS-resident launch (check: the last one's
 // plain-form residual reduced into dred), and whether a single-problem 2D solve runs its iterations that
 // way (small grids)
@@ -543,7 +552,7 @@ struct BatchMember {
     int64_t iter;         // PT iterations of the current physical step
     int64_t itmax;
     int32_t ndim, dcase, arith, nout;
-    int32_t lpb;          // sweep launches per check block (3D: nout / 2 two-level launches; 2D: nout)
+    int32_t lpb;          // sweep launches per check block (each flips the H / q buffers)
     int32_t active;       // 0 once every physical step is done (or the error is NaN)
     int32_t hpar, qpar;   // H / q buffer holding the current iterate at the start of a check block
     int32_t assign;       // set by the controller: Hold <- H before the next block (the step ended)
@@ -578,6 +587,12 @@ int check_chunk_tz(const ptd_params& p);    // ... of their checking launches (t
 hipError_t launch_batch_sweep(hipStream_t s, int kind, int dcase, int arith, int rows, bool check,
                               BatchMember* mem, const BatchTile* tiles, int ntiles, int li);
 int batch_rows_nocheck(int kind, int dcase, int arith, int variant);  // rows of the non-checking launch
+// three-level passes over the tiles of a group's members (float64 FAST linear / step groups, batch_t3),
+// rows fused3_rows() per block
+hipError_t launch_batch_sweep3(hipStream_t s, int dcase, int arith, BatchMember* mem, const BatchTile* tiles,
+                               int ntiles, int li);
+bool batch_t3(const ptd_params& p);
+int fused3_rows();
 int batch_rows_check(int kind, int dcase, int arith);                 // rows of the checking launch
 hipError_t launch_batch_reduce(hipStream_t s, BatchMember* mem, const int32_t* members, int n);
 // the reference loop's rules for the members of one group after a check block; *active_count (may be null)

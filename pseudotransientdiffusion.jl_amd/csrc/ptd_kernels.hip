@@ -1877,12 +1877,20 @@ hipError_t launch_fused3(ptd_ctx* c) {
 // 1.235 / 1.380; EXACT lin 1.44 / 1.66, step 2.021 / 2.019, nonlinear 2.21 / 2.42; float32 nonlinear
 // 0.653 / 0.907, lin 0.615 / 0.731 (the third level's halo work and registers cost more than its bytes
 // save). PTD_T3=0 / 1 in the environment switches them off / on for every 3D grid.
-bool fused3_available(const ptd_ctx* c) {
-    if (!(c->p.ndim == 3 && fused_available(c) && max_chunk_tz(c->p, (int64_t)c->esz) >= 2)) return false;
+bool fused3_wanted(const ptd_params& p, int64_t esz) {
+    if (!(p.ndim == 3 && p.nx >= 3 && p.ny >= 3 && p.nz >= 3 && max_chunk_tz(p, esz) >= 2)) return false;
     const char* env = getenv("PTD_T3");
     if (env && env[0] == '0') return false;
     if (env && env[0] == '1') return true;
-    return c->esz == 8 && c->p.arith == PTD_ARITH_FAST && (c->p.dcase == PTD_LIN || c->p.dcase == PTD_STEP);
+    return esz == 8 && p.arith == PTD_ARITH_FAST && (p.dcase == PTD_LIN || p.dcase == PTD_STEP);
+}
+bool fused3_available(const ptd_ctx* c) {
+    return fused_available(c) && fused3_wanted(c->p, (int64_t)c->esz);
+}
+int next_launch_iters(int64_t n, bool check_last, bool t2, bool t3) {
+    if (t3 && n - (check_last ? 2 : 0) >= 3 && !(check_last && n == 6)) return 3;
+    if (t2 && n >= 2) return 2;
+    return 1;
 }
 int lds2d_max_iters() { return kLdsHalo; }
 
@@ -2711,6 +2719,30 @@ k_batch3d_t2(BatchMember* __restrict__ mem, const BatchTile* __restrict__ tiles,
         as_global(M.q[2][qp ^ 1]), as_global(M.partial + tile), tile % ntx, rest % nty, rest / nty, tz);
 }
 
+// three-level passes of the members of a float64 FAST linear / step group (no residual)
+template <int DC, int ARITH>
+__global__ void __launch_bounds__(64 * kRows3)
+k_batch3d_t3(BatchMember* __restrict__ mem, const BatchTile* __restrict__ tiles, int ntot, int per_xcd, int li) {
+    constexpr int R = kRows3;
+    const int b = blockIdx.x;
+    const int t = (b & 7) * per_xcd + (b >> 3);
+    if (t >= ntot) return;
+    const BatchTile e = tiles[t];
+    const int mi = __builtin_amdgcn_readfirstlane(e.member);
+    const int tile = __builtin_amdgcn_readfirstlane(e.tile);
+    const int tz = __builtin_amdgcn_readfirstlane(e.tz);
+    const BatchMember& M = mem[mi];
+    if (!M.active) return;
+    const int ntx = ((int)M.p.nx - 2 + 57) / 58;
+    const int nty = ((int)M.p.ny - 2 + R - 7) / (R - 6);
+    const int hp = (M.hpar + li) & 1, qp = (M.qpar + li) & 1;
+    const int rest = tile / ntx;
+    fused3d_t3_tile<double, DC, PTD_PLAIN, ARITH, R>(
+        M.p, as_global(M.H[hp]), as_global(M.H[hp ^ 1]), as_global(M.Hold), as_global(M.D), as_global(M.q[0][qp]),
+        as_global(M.q[1][qp]), as_global(M.q[2][qp]), as_global(M.q[0][qp ^ 1]), as_global(M.q[1][qp ^ 1]),
+        as_global(M.q[2][qp ^ 1]), tile % ntx, rest % nty, rest / nty, tz);
+}
+
 template <int DC, int ARITH, int R, bool CHECK>
 __global__ void __launch_bounds__(64 * R)
 k_batch2d(BatchMember* __restrict__ mem, const BatchTile* __restrict__ tiles, int ntot, int per_xcd, int li) {
@@ -2822,6 +2854,29 @@ hipError_t launch_batch_sweep(hipStream_t s, int kind, int dcase, int arith, int
             return fast ? batch_sweep_t<PTD_NONLIN, kFast>(s, kind, rows, check, mem, tiles, ntiles, li)
                         : batch_sweep_t<PTD_NONLIN, kExact>(s, kind, rows, check, mem, tiles, ntiles, li);
     }
+}
+
+// Three-level passes of a batched group (float64 FAST linear / step members; other groups run two-level
+// passes only).
+bool batch_t3(const ptd_params& p) {
+    return p.ndim == 3 && p.arith == PTD_ARITH_FAST && (p.dcase == PTD_LIN || p.dcase == PTD_STEP) &&
+           fused3_wanted(p, 8);
+}
+int fused3_rows() { return kRows3; }
+
+hipError_t launch_batch_sweep3(hipStream_t s, int dcase, int arith, BatchMember* mem, const BatchTile* tiles,
+                               int ntiles, int li) {
+    if (ntiles <= 0) return hipSuccess;
+    if (arith != PTD_ARITH_FAST) return hipErrorInvalidValue;
+    const int per = (ntiles + 7) / 8;
+    const dim3 g(8 * per), bl(64, kRows3);
+    if (dcase == PTD_LIN)
+        hipLaunchKernelGGL((k_batch3d_t3<PTD_LIN, kFast>), g, bl, 0, s, mem, tiles, ntiles, per, li);
+    else if (dcase == PTD_STEP)
+        hipLaunchKernelGGL((k_batch3d_t3<PTD_STEP, kFast>), g, bl, 0, s, mem, tiles, ntiles, per, li);
+    else
+        return hipErrorInvalidValue;
+    return hipGetLastError();
 }
 
 // per member of a group: (sum, max) of its checking launch's tile partials, in the fixed order of the

@@ -57,11 +57,8 @@ def _setup(pkg, arith):
     return params, fields
 
 
-@pytest.mark.parametrize("arith", [0, 1], ids=["exact", "fast"])
-def test_batch_equals_single_solves(arith, pkg):
-    print(f"[gpu] batch vs single solves, arith={arith}", flush=True)
-    params, fields = _setup(pkg, arith)
-    nt = 5
+def _batch_vs_single(pkg, params, fields, nt=5):
+    """Solve the members as one batch (two calls), then one at a time, and require equal bits."""
     with pkg.Batch(params) as b:
         for m, (H0, D) in enumerate(fields):
             b.set(m, "H", H0)
@@ -93,6 +90,32 @@ def test_batch_equals_single_solves(arith, pkg):
                 assert np.array_equal(bHo[m], ctx.get("Hold")), f"member {m} call {call}: Hold"
                 for a, f in enumerate(("qHx", "qHy", "qHz")[:params[m].ndim]):
                     assert np.array_equal(bq[m][a], ctx.get(f)), f"member {m} call {call}: {f}"
+
+
+@pytest.mark.parametrize("arith", [0, 1], ids=["exact", "fast"])
+def test_batch_equals_single_solves(arith, pkg):
+    print(f"[gpu] batch vs single solves, arith={arith}", flush=True)
+    params, fields = _setup(pkg, arith)
+    _batch_vs_single(pkg, params, fields)
+
+
+@pytest.mark.parametrize("nout", [6, 12, 14])
+def test_batch_check_blocks_of_other_lengths(nout, pkg):
+    """Check blocks of other lengths: the float64 FAST linear and step members take three-level passes,
+    then two-level ones, and end every block on the checking two-level pass, in the batch and in their
+    single solves alike (nout = 6 and 12 would otherwise end a single solve's block on a single-level
+    checking sweep, whose residual partials are summed in another order)."""
+    print(f"[gpu] batch vs single solves, nout={nout}", flush=True)
+    params, fields = [], []
+    for script, shape, amp, centre in MEMBERS[:6]:
+        spec = pkg.SCRIPTS[script]
+        p = pkg.make_params(spec, *shape, arith=1)
+        p.nout = nout
+        p.itmax = nout * 2000
+        params.append(p)
+        fields.append((_gauss(spec, shape, amp, centre),
+                       pkg.step_field(spec, *shape) if spec["dcase"] == pkg.STEP else None))
+    _batch_vs_single(pkg, params, fields)
 
 
 def test_batch_workload_sample(pkg):
