@@ -14,6 +14,8 @@ and write H over the r inner cells, read and write the three flux arrays of q fa
 launch time; frac = achieved / 8 TB/s. traffic = the launch's measured HBM bytes (rocprofv3 FETCH_SIZE +
 WRITE_SIZE, committed profile), traffic_frac = traffic / launch time / 8 TB/s. teff_over_peak = T_eff /
 8 TB/s, which counts the reference's A_eff for each of the launch's two iterations and can exceed 1.
+stream_copy_GBs = a plain 2 GiB device copy timed in the same run (the box's achievable HBM rate), with
+achieved_over_stream and traffic_over_stream beside the fractions of the published peak.
 
 Extra legs (N=1), each in its own object of the JSON line:
  - time_to_solution: the full converging 5-step solve (tol 1e-8, check every nout=10 iterations) in
@@ -83,6 +85,28 @@ def calib_copy():
     torch.cuda.synchronize()
     print(json.dumps({"calib_copy_bytes_read": n * 8, "calib_copy_bytes_written": n * 8, "launches": 3}))
     return 0
+
+
+def stream_copy_gbs(repeats=10):
+    """The box's achievable HBM rate: a plain device-to-device copy of 2 GiB (read 2 GiB + write 2 GiB per
+    copy, torch's copy kernel), timed with events over `repeats` copies after three untimed ones. The roofline
+    reports the sweep's rates against it beside the 8 TB/s peak."""
+    import torch
+    n = 1 << 28
+    a = torch.ones(n, dtype=torch.float64, device="cuda")
+    b = torch.empty_like(a)
+    for _ in range(3):
+        b.copy_(a)
+    e0, e1 = torch.cuda.Event(enable_timing=True), torch.cuda.Event(enable_timing=True)
+    e0.record()
+    for _ in range(repeats):
+        b.copy_(a)
+    e1.record()
+    e1.synchronize()
+    ms = e0.elapsed_time(e1) / repeats
+    del a, b
+    torch.cuda.empty_cache()
+    return 2 * n * 8 / (ms / 1e3) / 1e9
 
 
 def measured_traffic(n, dtype, arith):
@@ -502,6 +526,14 @@ def main():
         out["roofline"]["traffic_GBs"] = round(traffic / launch_s / 1e9, 2)
         out["roofline"]["traffic_frac"] = round(traffic / launch_s / 1e9 / PEAK_HBM_GBS, 4)
         out["roofline"]["traffic_over_compulsory"] = round(traffic / comp, 4)
+    if torch is not None:
+        # the same rates against what a plain copy reaches on this box
+        stream = stream_copy_gbs()
+        log(f"stream copy {stream:.1f} GB/s")
+        out["roofline"]["stream_copy_GBs"] = round(stream, 2)
+        out["roofline"]["achieved_over_stream"] = round(achieved / stream, 4)
+        if traffic:
+            out["roofline"]["traffic_over_stream"] = round(traffic / launch_s / 1e9 / stream, 4)
 
     if rep_ms:
         med = sorted(rep_ms)[len(rep_ms) // 2]
