@@ -421,6 +421,9 @@ int32_t ptd_batch_create(const ptd_params* params, int32_t n, int32_t device, pt
             g->arith = p.arith;
             g->plan = block_plan(p);  // a function of (kind, dcase, arith) and the common nout
         }
+        if (block_plan(p) != g->plan)  // one graph serves the group: its members share the launch plan
+            return fail_free(ptd::set_error(PTD_ERR_ARG, "member " + std::to_string(m) +
+                                                             ": check-block plan differs from its group's"));
         g->members.push_back(m);
     }
     for (auto& g : b->groups) g.sub = group_sub_blocks(b->params, g.members);
