@@ -258,7 +258,7 @@ int32_t ptd_batch_solve(ptd_batch* batch, int32_t nt, int64_t* iters, double* er
 int32_t ptd_batch_tiling(const ptd_batch* batch, int32_t* tz3d, int32_t* rows2d, double* candidate_ms, int32_t ncand);
 /* Where the last ptd_batch_solve spent its time, per member group (members sharing a sweep kind, D case and
  * arithmetic run as one launch per sweep): *ngroups, and for the first `cap` groups the sweep kind (3: 3D
- * two-level, 22: 2D two-level, 2: 2D single-level), the D case, the member count and the device time the
+ * two- / three-level, 22: 2D two-level, 20: 2D LDS-resident, 2: 2D single-level), the D case, the member count and the device time the
  * group's stream took, summed over the replays (ms; the groups run concurrently, so the times overlap). */
 int32_t ptd_batch_group_times(const ptd_batch* batch, int32_t* ngroups, int32_t* kind, int32_t* dcase,
                               int32_t* members, double* ms, int32_t cap);

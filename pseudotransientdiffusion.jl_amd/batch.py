@@ -89,14 +89,14 @@ class Batch:
         return iters, errs, lat, status, total.value
 
     def group_times(self):
-        """Per member group of the last solve: dicts with the sweep kind ("3D", "2D two-level", "2D"), the D
+        """Per member group of the last solve: dicts with the sweep kind ("3D", "2D two-level", "2D LDS", "2D"), the D
         case, the member count and the device time the group's stream took (ms, summed over replays)."""
         cap = 64
         ng = ctypes.c_int32()
         kind, dcase, mem = (ctypes.c_int32 * cap)(), (ctypes.c_int32 * cap)(), (ctypes.c_int32 * cap)()
         ms = (ctypes.c_double * cap)()
         check(self._lib.ptd_batch_group_times(self._h, ctypes.byref(ng), kind, dcase, mem, ms, cap))
-        names = {3: "3D", 22: "2D two-level", 2: "2D"}
+        names = {3: "3D", 22: "2D two-level", 20: "2D LDS", 2: "2D"}
         dnames = {0: "lin", 1: "step", 2: "H^3"}
         return [{"kind": names.get(kind[g], str(kind[g])), "D": dnames.get(dcase[g], str(dcase[g])),
                  "members": int(mem[g]), "ms": round(ms[g], 2)} for g in range(min(ng.value, cap))]

@@ -5,8 +5,9 @@
 // the same checking tiling (so the same residual partials and stopping decisions), the same
 // iteration counts and the same bits. What changes is the schedule:
 //  - members are grouped by (sweep kind, dcase, arith), the kind being the sweep ptd_solve runs for
-//    them (two-level 3D, two-level 2D on large grids, single-level 2D); one launch covers the tiles of
-//    every member of a group (a table maps each block to its member, tile and z chunk depth);
+//    them (three- and two-level 3D, LDS-resident 2D on grids of at most 1024 cores, two-level 2D on
+//    larger ones, single-level 2D); one launch covers the tiles of every member of a group (a table maps
+//    each block to its member, tile and z chunk depth);
 //  - the PT loop of every member runs on the device: after each check block of nout iterations the
 //    group's controller kernel applies the reference loop's rules per member (scripts/diff_3D/
 //    diff_3D_nonlin_multixpu.jl:104-118) and members that are done drop out of later launches;
@@ -94,11 +95,22 @@ int group_sub_blocks(const std::vector<ptd_params>& params, const std::vector<in
 // use the single-problem tiling (its residual partials decide when a member stops); every other launch
 // gives the same bits whatever its depth or tiling. 3D members follow the single-problem rule
 // (ptd::next_launch_iters: three-level passes for the float64 FAST linear / step members, two-level
-// pairs, a checking two-level pass last), the two-level 2D groups run pairs, and the single-level 2D
-// groups (grids too small for ptd_solve to pick the two-level 2D sweep) run their first nout - 2
-// iterations as two-level pairs, then one single-level sweep and the single-problem checking launch.
+// pairs, a checking two-level pass last); 2D grids of at most 1024 cores run LDS-resident launches of up
+// to lds2d_max_iters() iterations as their single solves do (the last one checking); the two-level 2D
+// groups run pairs; and the single-level 2D groups (LDS-resident iterations switched off, grids too small
+// for the two-level 2D sweep) run their first nout - 2 iterations as two-level pairs, then one
+// single-level sweep and the single-problem checking launch.
 std::vector<int> block_plan(const ptd_params& p) {
     std::vector<int> plan;
+    if (ptd::batch_kind(p) == ptd::kSweepLds2D) {  // run_iterations' LDS-resident rule
+        const int64_t lmax = ptd::lds2d_max_iters();
+        for (int64_t n = p.nout; n > 0;) {
+            const int64_t k = n <= lmax ? n : std::min<int64_t>(n - 1, lmax);
+            plan.push_back((int)k);
+            n -= k;
+        }
+        return plan;
+    }
     if (ptd::batch_kind(p) == ptd::kSweep2D) {
         for (int i = 0; i < p.nout / 2 - 1; ++i) plan.push_back(2);
         plan.push_back(1);
@@ -232,6 +244,7 @@ int member_tiles3(const ptd_params& p, int tz) {
 
 // tiles of one member's launch: rows per block (two-level 2D: rows per tile chunk), z chunk depth tz (3D)
 int member_tiles(const ptd_params& p, int kind, int rows, int tz) {
+    if (kind == ptd::kSweepLds2D) return (int)ptd::lds2d_tiles(p);
     if (kind == ptd::kSweep2D2) return (int)(((p.nx - 2 + 59) / 60) * ((p.ny - 2 + rows - 1) / rows));
     if (p.ndim == 3) {
         const int64_t ntx = (p.nx - 2 + 59) / 60, nty = (p.ny - 2 + rows - 5) / (rows - 4);
@@ -263,6 +276,11 @@ int enqueue_group(ptd_batch* b, size_t gi, int cand) {
     for (int sb = 0; sb < g.sub; ++sb) {
         for (int li = 0; li < lpb; ++li) {
             const bool check = li == lpb - 1;
+            if (g.kind == ptd::kSweepLds2D) {  // LDS-resident iterations (one tiling: the checking one)
+                PTD_HIPB(ptd::launch_batch_lds2d(g.stream, g.dcase, g.arith, b->dm, g.d_check, g.n_check, li,
+                                                 g.plan[(size_t)li], check));
+                continue;
+            }
             if (g.plan[(size_t)li] == 3) {  // a three-level pass
                 PTD_HIPB(ptd::launch_batch_sweep3(g.stream, g.dcase, g.arith, b->dm, g.d_t3[cand], g.n_t3[cand], li));
                 continue;
@@ -744,7 +762,8 @@ int32_t ptd_batch_group_times(const ptd_batch* b, int32_t* ngroups, int32_t* kin
         uint64_t span[2] = {0, 0};
         if (hipMemcpy(span, G.d_span, sizeof(span), hipMemcpyDeviceToHost) != hipSuccess)
             return ptd::set_error(PTD_ERR_HIP, "reading the group clocks failed");
-        if (kind) kind[g] = G.kind == ptd::kSweep3D ? 3 : G.kind == ptd::kSweep2D2 ? 22 : 2;
+        if (kind)
+            kind[g] = G.kind == ptd::kSweep3D ? 3 : G.kind == ptd::kSweep2D2 ? 22 : G.kind == ptd::kSweepLds2D ? 20 : 2;
         if (dcase) dcase[g] = G.dcase;
         if (members) members[g] = (int32_t)G.members.size();
         if (ms) ms[g] = b->wall_khz > 0 ? (double)span[1] / b->wall_khz : 0.0;

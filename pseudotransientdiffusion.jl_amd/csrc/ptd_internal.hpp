@@ -574,9 +574,11 @@ enum { kBatchMaxMembers = 1024 };
 int validate_params(const ptd_params* p);          // the single-problem checks (ptd_api.hip)
 int set_error(int code, const std::string& msg);   // sets ptd_last_error(), returns code
 
-// The sweep a problem runs, as ptd_solve picks it: kSweep3D (two-level 3D), kSweep2D2 (two-level 2D,
-// grids with enough tiles to fill the GPU) or kSweep2D (single-level 2D).
-enum { kSweep2D = 2, kSweep3D = 3, kSweep2D2 = 4 };
+// The sweep a problem runs, as ptd_solve picks it: kSweep3D (two- / three-level 3D), kSweepLds2D (2D grids
+// of at most 1024 32 x 32 cores, LDS-resident iterations), kSweep2D2 (two-level 2D, grids with enough
+// tiles to fill the GPU) or kSweep2D (single-level 2D).
+enum { kSweep2D = 2, kSweep3D = 3, kSweep2D2 = 4, kSweepLds2D = 5 };
+bool use_2d_lds_params(const ptd_params& p);  // whether 2D solves of p run LDS-resident iterations
 bool use_2d_t2(const ptd_params& p);
 int batch_kind(const ptd_params& p);
 int default_chunk_tz(const ptd_params& p);  // z chunk depth of the single-problem 3D sweeps
@@ -593,6 +595,10 @@ hipError_t launch_batch_sweep3(hipStream_t s, int dcase, int arith, BatchMember*
                                int ntiles, int li);
 bool batch_t3(const ptd_params& p);
 int fused3_rows();
+// niter (1..lds2d_max_iters()) LDS-resident PT iterations over the 32 x 32 cores of a 2D group's members
+// (the checking launch also writes each core's residual partial)
+hipError_t launch_batch_lds2d(hipStream_t s, int dcase, int arith, BatchMember* mem, const BatchTile* tiles,
+                              int ntiles, int li, int niter, bool check);
 int batch_rows_check(int kind, int dcase, int arith);                 // rows of the checking launch
 hipError_t launch_batch_reduce(hipStream_t s, BatchMember* mem, const int32_t* members, int n);
 // the reference loop's rules for the members of one group after a check block; *active_count (may be null)

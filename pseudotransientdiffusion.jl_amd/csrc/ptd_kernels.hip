@@ -721,11 +721,14 @@ static constexpr int kLdsHalo = 10;
 static constexpr int kLdsW = kLdsCore + 2 * kLdsHalo;  // 52
 static constexpr int kLdsThreads = 1024;
 
+// One 32 x 32 core (tx, ty) and its frame through niter PT iterations; `part` receives the core's residual
+// partial (CHECK). Called by the single-problem kernel below and by the batched kernel.
 template <typename T, int DC, int FORM, int ARITH, bool CHECK>
-__global__ void __launch_bounds__(kLdsThreads)
-k_lds2d(const KP<T> p, const T* __restrict__ Hin, T* __restrict__ Hout, const T* __restrict__ Hold,
-        const T* __restrict__ D, const T* __restrict__ qxi, const T* __restrict__ qyi, T* __restrict__ qxo,
-        T* __restrict__ qyo, int niter, int ntx, int ntiles, int per_xcd, double2* __restrict__ partial) {
+__device__ __forceinline__ void lds2d_tile(const KP<T>& p, const T* __restrict__ Hin, T* __restrict__ Hout,
+                                           const T* __restrict__ Hold, const T* __restrict__ D,
+                                           const T* __restrict__ qxi, const T* __restrict__ qyi,
+                                           T* __restrict__ qxo, T* __restrict__ qyo, int niter, int tx, int ty,
+                                           double2* __restrict__ part) {
     using Ph = Phys<T, DC, FORM, ARITH>;
     constexpr bool STEP = DC == PTD_STEP;
     constexpr int W = kLdsW, NC = W * W;
@@ -736,12 +739,9 @@ k_lds2d(const KP<T> p, const T* __restrict__ Hin, T* __restrict__ Hout, const T*
     __shared__ T sRx[CHECK ? NC : 1];  // residual fluxes of the checking iteration
     __shared__ T sRy[CHECK ? NC : 1];
 
-    const int b = blockIdx.x;
-    const int tile = (b & 7) * per_xcd + (b >> 3);  // XCD-aware: each XCD walks a contiguous tile range
-    if (tile >= ntiles) return;
     const int nx = (int)p.nx, ny = (int)p.ny;
-    const int x0 = kLdsCore * (tile % ntx) - kLdsHalo;
-    const int y0 = kLdsCore * (tile / ntx) - kLdsHalo;
+    const int x0 = kLdsCore * tx - kLdsHalo;
+    const int y0 = kLdsCore * ty - kLdsHalo;
     const int t = threadIdx.x;
 
     T H[NS], Ho[NS], qx[NS], qy[NS], Dc[NS], Dw[NS], Ds[NS], dmax[NS];
@@ -854,7 +854,19 @@ k_lds2d(const KP<T> p, const T* __restrict__ Hin, T* __restrict__ Hout, const T*
         if (fys[s]) qyo[(int64_t)(i - 1) + (int64_t)(nx - 2) * (j - 1)] = qy[s];
     }
 
-    if constexpr (CHECK) block_sum_max(rs, rm, t, kLdsThreads, &partial[tile]);
+    if constexpr (CHECK) block_sum_max(rs, rm, t, kLdsThreads, part);
+}
+
+template <typename T, int DC, int FORM, int ARITH, bool CHECK>
+__global__ void __launch_bounds__(kLdsThreads)
+k_lds2d(const KP<T> p, const T* __restrict__ Hin, T* __restrict__ Hout, const T* __restrict__ Hold,
+        const T* __restrict__ D, const T* __restrict__ qxi, const T* __restrict__ qyi, T* __restrict__ qxo,
+        T* __restrict__ qyo, int niter, int ntx, int ntiles, int per_xcd, double2* __restrict__ partial) {
+    const int b = blockIdx.x;
+    const int tile = (b & 7) * per_xcd + (b >> 3);  // XCD-aware: each XCD walks a contiguous tile range
+    if (tile >= ntiles) return;
+    lds2d_tile<T, DC, FORM, ARITH, CHECK>(p, Hin, Hout, Hold, D, qxi, qyi, qxo, qyo, niter, tile % ntx, tile / ntx,
+                                          &partial[tile]);
 }
 
 // ---------------------------------------------------------------------------------------------
@@ -1924,7 +1936,12 @@ bool use_2d_t2(const ptd_params& p) {
     const int64_t ntx = (p.nx - 2 + 59) / 60, nty = (p.ny - 2 + kTy2D - 1) / kTy2D;
     return ntx * nty >= 1024;
 }
-int batch_kind(const ptd_params& p) { return p.ndim == 3 ? kSweep3D : (use_2d_t2(p) ? kSweep2D2 : kSweep2D); }
+// the sweep a single solve runs for p: LDS-resident 2D iterations take precedence (run_iterations)
+int batch_kind(const ptd_params& p) {
+    if (p.ndim == 3) return kSweep3D;
+    if (use_2d_lds_params(p)) return kSweepLds2D;
+    return use_2d_t2(p) ? kSweep2D2 : kSweep2D;
+}
 
 // LDS-resident 2D iterations pay off while a launch of single sweeps is latency-bound: grids of at most
 // kLdsMaxTiles 32x32 cores. PTD_LDS2D=0 / 1 in the environment switches them off / on for every 2D grid.
@@ -1932,14 +1949,14 @@ static constexpr int64_t kLdsMaxTiles = 1024;
 int64_t lds2d_tiles(const ptd_params& p) {
     return ((p.nx + kLdsCore - 1) / kLdsCore) * ((p.ny + kLdsCore - 1) / kLdsCore);
 }
-bool use_2d_lds(const ptd_ctx* c) {
-    const ptd_params& p = c->p;
-    if (p.ndim != 2 || !fused_available(c)) return false;
+bool use_2d_lds_params(const ptd_params& p) {
+    if (p.ndim != 2 || p.nx < 3 || p.ny < 3) return false;
     const char* env = getenv("PTD_LDS2D");
     if (env && env[0] == '0') return false;
     if (env && env[0] == '1') return true;
     return lds2d_tiles(p) <= kLdsMaxTiles;
 }
+bool use_2d_lds(const ptd_ctx* c) { return fused_available(c) && use_2d_lds_params(c->p); }
 // PTD_T2=2 in the environment also runs the single-problem 2D solves on two-level sweeps below that
 // threshold (the same bits as single sweeps; lets the tests compare them with the oracle on small grids).
 // Batches keep the threshold.
@@ -2782,10 +2799,56 @@ k_batch2d_t2(BatchMember* __restrict__ mem, const BatchTile* __restrict__ tiles,
         tile % ntx, tile / ntx, ty);
 }
 
+// up to kLdsHalo PT iterations of the members of a 2D group whose grids run LDS-resident in their single
+// solves, 32 x 32 cores per block; the checking launch also writes each core's residual partial
+template <int DC, int ARITH, bool CHECK>
+__global__ void __launch_bounds__(kLdsThreads)
+k_batch_lds2d(BatchMember* __restrict__ mem, const BatchTile* __restrict__ tiles, int ntot, int per_xcd, int li,
+              int niter) {
+    const int b = blockIdx.x;
+    const int t = (b & 7) * per_xcd + (b >> 3);
+    if (t >= ntot) return;
+    const BatchTile e = tiles[t];
+    const int mi = __builtin_amdgcn_readfirstlane(e.member);
+    const int tile = __builtin_amdgcn_readfirstlane(e.tile);
+    const BatchMember& M = mem[mi];
+    if (!M.active) return;
+    const int ntx = ((int)M.p.nx + kLdsCore - 1) / kLdsCore;
+    const int hp = (M.hpar + li) & 1, qp = (M.qpar + li) & 1;
+    lds2d_tile<double, DC, PTD_PLAIN, ARITH, CHECK>(
+        M.p, as_global(M.H[hp]), as_global(M.H[hp ^ 1]), as_global(M.Hold), as_global(M.D), as_global(M.q[0][qp]),
+        as_global(M.q[1][qp]), as_global(M.q[0][qp ^ 1]), as_global(M.q[1][qp ^ 1]), niter, tile % ntx, tile / ntx,
+        as_global(M.partial + tile));
+}
+
+hipError_t launch_batch_lds2d(hipStream_t s, int dcase, int arith, BatchMember* mem, const BatchTile* tiles,
+                              int ntiles, int li, int niter, bool check) {
+    if (ntiles <= 0) return hipSuccess;
+    if (niter < 1 || niter > kLdsHalo) return hipErrorInvalidValue;
+    const int per = (ntiles + 7) / 8;
+    const dim3 g(8 * per), bl(kLdsThreads);
+    const bool fast = arith == PTD_ARITH_FAST;
+#define PTD_BATCH_LDS(DC)                                                                                           \
+    if (fast && check) hipLaunchKernelGGL((k_batch_lds2d<DC, kFast, true>), g, bl, 0, s, mem, tiles, ntiles, per, li, niter); \
+    else if (fast) hipLaunchKernelGGL((k_batch_lds2d<DC, kFast, false>), g, bl, 0, s, mem, tiles, ntiles, per, li, niter);   \
+    else if (check) hipLaunchKernelGGL((k_batch_lds2d<DC, kExact, true>), g, bl, 0, s, mem, tiles, ntiles, per, li, niter);  \
+    else hipLaunchKernelGGL((k_batch_lds2d<DC, kExact, false>), g, bl, 0, s, mem, tiles, ntiles, per, li, niter);
+    if (dcase == PTD_LIN) {
+        PTD_BATCH_LDS(PTD_LIN)
+    } else if (dcase == PTD_STEP) {
+        PTD_BATCH_LDS(PTD_STEP)
+    } else {
+        PTD_BATCH_LDS(PTD_NONLIN)
+    }
+#undef PTD_BATCH_LDS
+    return hipGetLastError();
+}
+
 // rows of the batched launches: the checking launch uses the single-problem checking tiling (so the
 // residual partials, and hence every stopping decision, are those of a single-problem solve); the
 // other launches may use any tiling (every cell is computed by its owner with the same expressions).
 int batch_rows_check(int kind, int dcase, int arith) {
+    if (kind == kSweepLds2D) return kLdsCore;
     if (kind == kSweep2D2) return kTy2D;
     if (kind != kSweep3D) return kRows2D;
     const bool fast = arith == PTD_ARITH_FAST;
@@ -2794,6 +2857,7 @@ int batch_rows_check(int kind, int dcase, int arith) {
     return fast ? t2_rows<double, PTD_NONLIN, kFast, true>() : t2_rows<double, PTD_NONLIN, kExact, true>();
 }
 int batch_rows_nocheck(int kind, int dcase, int arith, int variant) {
+    if (kind == kSweepLds2D) return kLdsCore;
     if (kind == kSweep2D2) return kTy2D;
     if (kind == kSweep2D) return variant == 1 ? 8 : 16;
     const bool fast = arith == PTD_ARITH_FAST;

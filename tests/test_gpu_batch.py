@@ -2,18 +2,13 @@
 (ptd_solve through a context): every member's H, Hold, fluxes, per-step iteration counts and errors
 must be identical, bit for bit, for 2D and 3D members of every D type, in EXACT and FAST arithmetic,
 over two consecutive solve calls (the batch replays one captured check-block graph many times; the
-single-problem solves run eagerly, on the batch's sweeps: PTD_LDS2D=0 keeps small 2D grids off the
-LDS-resident iterations, whose residual partials sum over other tiles)."""
+single-problem solves run eagerly). Batches run each member on the sweep its single solve picks: small
+2D grids LDS-resident (also with PTD_LDS2D=0, which switches that off in both, so that the single-level
+and two-level 2D batch sweeps are compared too), large 2D grids two-level, 3D grids three- and two-level."""
 import numpy as np
 import pytest
 
 pytestmark = pytest.mark.gpu
-
-
-@pytest.fixture(autouse=True)
-def _batch_sweeps(monkeypatch):
-    """Single-problem solves on the batch's own sweeps (small 2D grids otherwise run LDS-resident)."""
-    monkeypatch.setenv("PTD_LDS2D", "0")
 
 # (script, shape, amplitude, centre fraction)
 MEMBERS = [
@@ -24,8 +19,10 @@ MEMBERS = [
     ("diff_3D_linstep_multixpu", (30, 24, 20), 2.0, (0.5, 0.5, 0.5)),
     ("diff_3D_nonlin_multixpu", (28, 28, 28), 1.0, (0.45, 0.55, 0.5)),
     ("diff_3D_nonlin_multixpu", (67, 35, 21), 0.7, (0.3, 0.7, 0.5)),
-    # large enough (17 x 62 tiles) for the two-level 2D sweep
+    # LDS-resident in single solves (31 x 32 cores), two-level 2D sweep with PTD_LDS2D=0 (17 x 62 tiles)
     ("diff_2D_nonlin", (1000, 990), 1.0, (0.5, 0.45)),
+    # more than 1024 cores (35 x 32): the two-level 2D sweep (19 x 63 tiles)
+    ("diff_2D_lin", (1100, 1000), 0.9, (0.45, 0.5)),
     # the smallest grids: one inner cell / one inner row
     ("diff_3D_lin_multixpu", (3, 3, 3), 1.0, (0.5, 0.5, 0.5)),
     ("diff_2D_nonlin", (3, 4), 1.2, (0.5, 0.5)),
@@ -92,9 +89,14 @@ def _batch_vs_single(pkg, params, fields, nt=5):
                     assert np.array_equal(bq[m][a], ctx.get(f)), f"member {m} call {call}: {f}"
 
 
+@pytest.mark.parametrize("lds", ["default", "off"])
 @pytest.mark.parametrize("arith", [0, 1], ids=["exact", "fast"])
-def test_batch_equals_single_solves(arith, pkg):
-    print(f"[gpu] batch vs single solves, arith={arith}", flush=True)
+def test_batch_equals_single_solves(arith, lds, pkg, monkeypatch):
+    if lds == "off":
+        monkeypatch.setenv("PTD_LDS2D", "0")
+    else:
+        monkeypatch.delenv("PTD_LDS2D", raising=False)
+    print(f"[gpu] batch vs single solves, arith={arith}, LDS-resident 2D {lds}", flush=True)
     params, fields = _setup(pkg, arith)
     _batch_vs_single(pkg, params, fields)
 
