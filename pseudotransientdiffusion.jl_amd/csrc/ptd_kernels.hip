@@ -1328,7 +1328,14 @@ __device__ __forceinline__ void fused3d_t3_tile(const KP<T>& p, const T* __restr
     }
     T unused = 0;
     T q1zL = Ph::template face<false>(p, ldc<T>(bqz, oQz), ldc<T>(bHin, oH), Hc, Dm1, Dc, p.dz, p._dz, p._dz8, unused);
+    // operands two planes ahead: a three-level plane step does half as much again of arithmetic as a
+    // two-level one between its loads, and one plane of lookahead left them exposed (float64 FAST at 511^3,
+    // ms per iteration: linear 0.998 -> 0.836, step 1.236 -> 1.200, nonlinear 1.370 -> 1.284; three planes
+    // ahead were slower again). The float64 EXACT step pass keeps one plane: it would spill.
+    constexpr bool kAhead2 = !(sizeof(T) == 8 && DC == PTD_STEP && ARITH == kExact);
     PlaneOps2<T, T> A = load_plane(ka);
+    PlaneOps2<T, T> B;
+    if constexpr (kAhead2) B = load_plane(ka + 1 <= nz - 1 ? ka + 1 : nz - 1);
 
     T H1p = 0, H2p = 0;          // H1 at plane k-1, H2 at plane k-2
     T Hop = 0, Hopp = 0;         // Hold at planes k-1, k-2
@@ -1457,7 +1464,12 @@ __device__ __forceinline__ void fused3d_t3_tile(const KP<T>& p, const T* __restr
             Dc = A.Dn;
         }
         Hc = A.Hn;
-        A = load_plane(k + 1 <= nz - 1 ? k + 1 : nz - 1);
+        if constexpr (kAhead2) {
+            A = B;
+            B = load_plane(k + 2 <= nz - 1 ? k + 2 : nz - 1);
+        } else {
+            A = load_plane(k + 1 <= nz - 1 ? k + 1 : nz - 1);
+        }
     }
 }
 
