@@ -1791,7 +1791,8 @@ static hipError_t sweep2d_lds_run(ptd_ctx* c, const SweepBufs& b, int niter, boo
 // 1.106 / 1.169 / 1.183 (the model's choice: 102, five chunks of the 509 inner planes).
 int t3_chunk_tz(const ptd_params& p, int64_t esz) {
     const int64_t mz = p.nz - 2;
-    const int64_t lim = std::max<int64_t>(1, std::min<int64_t>(mz, max_chunk_tz(p, esz) - 1));
+    // (deeper than 512 planes a chunk saves nothing measurable; the bound keeps the search short per launch)
+    const int64_t lim = std::max<int64_t>(1, std::min<int64_t>(std::min<int64_t>(mz, 512), max_chunk_tz(p, esz) - 1));
     const int64_t tiles = (p.nx - 2 + 57) / 58 * ((p.ny - 2 + kRows3 - 7) / (kRows3 - 6));
     const int64_t slots = esz == 8 ? 256 : 512;
     int64_t best = lim, best_cost = -1;
