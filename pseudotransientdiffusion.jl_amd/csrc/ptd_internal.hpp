@@ -589,7 +589,7 @@ int check_chunk_tz(const ptd_params& p);    // ... of their checking launches (t
 hipError_t launch_batch_sweep(hipStream_t s, int kind, int dcase, int arith, int rows, bool check,
                               BatchMember* mem, const BatchTile* tiles, int ntiles, int li);
 int batch_rows_nocheck(int kind, int dcase, int arith, int variant);  // rows of the non-checking launch
-// three-level passes over the tiles of a group's members (float64 FAST linear / step groups, batch_t3),
+// three-level passes over the tiles of a group's members (float64 FAST 3D groups, batch_t3),
 // rows fused3_rows() per block
 hipError_t launch_batch_sweep3(hipStream_t s, int dcase, int arith, BatchMember* mem, const BatchTile* tiles,
                                int ntiles, int li);

@@ -2773,7 +2773,7 @@ k_batch3d_t2(BatchMember* __restrict__ mem, const BatchTile* __restrict__ tiles,
         as_global(M.q[2][qp ^ 1]), as_global(M.partial + tile), tile % ntx, rest % nty, rest / nty, tz);
 }
 
-// three-level passes of the members of a float64 FAST linear / step group (no residual)
+// three-level passes of the members of a float64 FAST group (no residual)
 template <int DC, int ARITH>
 __global__ void __launch_bounds__(64 * kRows3)
 k_batch3d_t3(BatchMember* __restrict__ mem, const BatchTile* __restrict__ tiles, int ntot, int per_xcd, int li) {
@@ -2957,11 +2957,17 @@ hipError_t launch_batch_sweep(hipStream_t s, int kind, int dcase, int arith, int
     }
 }
 
-// Three-level passes of a batched group (float64 FAST linear / step members; other groups run two-level
-// passes only).
+// Three-level passes of a batched group: the float64 FAST 3D groups. Linear and step members take them as
+// their single solves do; nonlinear members take them too, as a batch's non-checking launches may use any
+// pass (the checking one keeps the single-problem tiling): at the batch's grid sizes the nonlinear
+// three-level pass is the faster one up to ~224^3 (ms per iteration, two- / three-level: 128^3 0.0356 /
+// 0.0312, 224^3 0.133 / 0.121; 256^3 0.175 / 0.200), and the 96-member wall went 4.234 -> 4.212 s
+// (medians, same box). PTD_T3=0 switches them off.
 bool batch_t3(const ptd_params& p) {
-    return p.ndim == 3 && p.arith == PTD_ARITH_FAST && (p.dcase == PTD_LIN || p.dcase == PTD_STEP) &&
-           fused3_wanted(p, 8);
+    if (p.ndim != 3 || p.arith != PTD_ARITH_FAST) return false;
+    if (p.dcase == PTD_LIN || p.dcase == PTD_STEP) return fused3_wanted(p, 8);
+    const char* env = getenv("PTD_T3");
+    return !(env && env[0] == '0') && p.nx >= 3 && p.ny >= 3 && p.nz >= 3 && max_chunk_tz(p, 8) >= 2;
 }
 int fused3_rows() { return kRows3; }
 
@@ -2976,7 +2982,7 @@ hipError_t launch_batch_sweep3(hipStream_t s, int dcase, int arith, BatchMember*
     else if (dcase == PTD_STEP)
         hipLaunchKernelGGL((k_batch3d_t3<PTD_STEP, kFast>), g, bl, 0, s, mem, tiles, ntiles, per, li);
     else
-        return hipErrorInvalidValue;
+        hipLaunchKernelGGL((k_batch3d_t3<PTD_NONLIN, kFast>), g, bl, 0, s, mem, tiles, ntiles, per, li);
     return hipGetLastError();
 }
 
