@@ -113,3 +113,32 @@ def test_t3_solve_with_checks_matches_oracle(script, pkg, orc, monkeypatch):
     assert list(res.iters) == list(ref["iters"])
     assert np.array_equal(res.H, ref["H"]), f"rel {_relmax(res.H, ref['H']):.2e}"
     np.testing.assert_allclose(res.errs, ref["errs"], rtol=1e-12)
+
+
+@pytest.mark.parametrize("script", ["diff_3D_lin_multixpu", "diff_3D_linstep_multixpu"])
+def test_t3_full_size_equals_two_level(script, pkg, monkeypatch):
+    """At 511^3 the three-level passes take the depth of their own chunk model (102 planes: five chunks of the
+    509 inner planes, offsets up to 108 planes from a chunk's base): 9 FAST iterations (three passes) give the
+    same bits as the two-level sweeps over the same iterations, on the benchmark IC (step case: D = 1e-4 for
+    x < lx/2)."""
+    n, niter = 511, 9
+    print(f"[gpu] t3 full size {script} {n}^3", flush=True)
+    monkeypatch.delenv("PTD_TZ", raising=False)
+    spec = pkg.SCRIPTS[script]
+    params = pkg.make_params(spec, n, n, n, arith=1)
+    H0 = pkg.initial_condition(spec, n, n, n, kind="config")
+    D = pkg.step_field(spec, n, n, n, kind="half_x") if spec["dcase"] == pkg.STEP else None
+    out = []
+    for t3 in ("1", "0"):
+        monkeypatch.setenv("PTD_T3", t3)
+        with pkg.Context(params) as ctx:
+            ctx.set("H", H0)
+            ctx.set("Hold", H0)
+            if D is not None:
+                ctx.set("D", D)
+            ctx.iterate(niter)
+            out.append((ctx.get("H"), ctx.get("qHz")))
+    assert np.all(np.isfinite(out[0][0]))
+    assert _relmax(out[0][0], H0) > 1e-9  # the sweeps moved the field
+    assert np.array_equal(out[0][0], out[1][0]), f"H: rel {_relmax(out[0][0], out[1][0]):.2e}"
+    assert np.array_equal(out[0][1], out[1][1])
