@@ -389,9 +389,11 @@ def summary_lines(name, result, n):
 
 
 def main(argv=None):
-    """CLI mirror of running a script: `python -m <pkg>.scripts diff_3D_nonlin_perf` with NX/NY/NZ, DO_SAVE
-    and DO_SAVE_VIZ from the environment (scripts/diff_3D/diff_3D_nonlin_perf.jl:1-8): prints the script's
-    summary lines, appends its row to output/<text_file_name> and writes out_visu/<viz_file_name>."""
+    """CLI mirror of running a script (`python __graft_entry__.py script diff_3D_nonlin_perf` from the
+    repository root) with NX/NY/NZ, DO_SAVE and DO_SAVE_VIZ from the environment
+    (scripts/diff_3D/diff_3D_nonlin_perf.jl:1-8): prints the script's summary lines, appends its row to
+    output/<text_file_name> and writes out_visu/<viz_file_name>. PTD_ARITH=fast runs the library's FAST
+    arithmetic instead of the reference's operation order (exact, the default)."""
     argv = sys.argv[1:] if argv is None else argv
     if not argv or argv[0] not in SCRIPTS:
         print("usage: scripts.py <" + "|".join(SCRIPTS) + ">", file=sys.stderr)
@@ -404,7 +406,11 @@ def main(argv=None):
     ny = int(os.environ.get("NY", default_n))
     nz = int(os.environ.get("NZ", default_n))
     n = [nx, ny, nz][:nd]
-    r = run_script(name, nx, ny, nz)
+    arith = os.environ.get("PTD_ARITH", "exact").strip().lower()
+    if arith not in ("exact", "fast"):
+        print("PTD_ARITH must be exact or fast", file=sys.stderr)
+        return 2
+    r = run_script(name, nx, ny, nz, arith=_abi.ARITH_FAST if arith == "fast" else _abi.ARITH_EXACT)
     for line in summary_lines(name, r, n):
         print(line)
     if _env_bool("DO_SAVE", False) and not spec.get("react"):
