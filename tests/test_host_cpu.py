@@ -246,3 +246,16 @@ def test_main_output_rows(name, tmp_path, monkeypatch, capsys):
         xc = S.linrange(10.0 / 64, 10.0 - 10.0 / 64, 32)
         hana = 1 / np.sqrt(4 * 1.25) * np.exp(-(xc - 5.0) ** 2 / (4 * 1.25))
         assert m.group(1) == "%1.2e" % (np.linalg.norm(0.5 - hana) / np.sqrt(32))
+
+
+def test_script_command_line_arguments(pkg, monkeypatch, capsys):
+    """The scripts' command line (`python __graft_entry__.py script <name>`) refuses an unknown script and a
+    PTD_ARITH other than exact / fast before it touches the device."""
+    from importlib import import_module
+    S = import_module("ptd_amd.scripts")
+    assert S.main([]) == 2
+    assert S.main(["diff_9D_lin"]) == 2
+    assert "usage" in capsys.readouterr().err
+    monkeypatch.setenv("PTD_ARITH", "fp16")
+    assert S.main(["diff_2D_lin"]) == 2
+    assert "PTD_ARITH" in capsys.readouterr().err
