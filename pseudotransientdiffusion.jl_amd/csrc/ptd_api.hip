@@ -255,9 +255,16 @@ int run_iterations(ptd_ctx* c, int64_t n, bool check_last) {
     return PTD_OK;
 }
 
-// float32 correction iterations (pairs through the two-level sweep), the last one optionally checked
+// float32 correction iterations (three-level passes for linear / step D on large grids, pairs through the
+// two-level sweep), the last one optionally checked, on a two-level pass as in run_iterations
 int run_delta(ptd_ctx* c, int64_t n, bool check_last) {
+    const bool t3 = ptd::delta_t3(c);
     while (n > 0) {
+        if (ptd::next_launch_iters(n, check_last, true, t3) == 3) {
+            PTD_HIP(ptd::launch_delta_sweep3(c));
+            n -= 3;
+            continue;
+        }
         const bool two = n >= 2;
         const bool chk = check_last && n == (two ? 2 : 1);
         PTD_HIP(ptd::launch_delta_sweep(c, two, chk));

@@ -492,7 +492,8 @@ bool fused2_available(const ptd_ctx* c);
 hipError_t launch_fused3(ptd_ctx* c);  // three fused PT iterations in one pass (3D, no residual)
 bool fused3_available(const ptd_ctx* c);
 // whether the 3D sweeps of a problem with these parameters and element size take three-level passes
-// (the float64 FAST linear and step sweeps; PTD_T3=0 / 1 in the environment overrides)
+// (the FAST linear and step sweeps: float64 on every grid, float32 on grids of at least kT3MinCellsF32
+// inner cells; PTD_T3=0 / 1 in the environment overrides)
 bool fused3_wanted(const ptd_params& p, int64_t esz);
 // PT iterations of the next launch of a 3D run of n iterations (3: a three-level pass, 2: a two-level
 // pass, 1: a single sweep), given which passes are available. With check_last the run ends on a checking
@@ -511,6 +512,10 @@ int64_t lds2d_tiles(const ptd_params& p);
 int64_t fused_check_tiles(const ptd_ctx* c);
 // mixed precision: float32 correction sweeps (1 or 2 iterations), source and correction kernels
 hipError_t launch_delta_sweep(ptd_ctx* c, bool two, bool check);
+// three float32 correction iterations in one three-level pass (linear and step D), and whether a mixed
+// solve takes them (fused3_wanted for its float32 FAST sweeps)
+hipError_t launch_delta_sweep3(ptd_ctx* c);
+bool delta_t3(const ptd_ctx* c);
 hipError_t launch_f64_to_f32(ptd_ctx* c, float* dst, const double* src, int64_t n);
 // correction fluxes <-> true fluxes: to_p: p = float(q + g(H)); !to_p: q = double(p) - g(H), where g(H)
 // is the equilibrium flux magnitude coef*dH/h of the float64 iterate H on each face

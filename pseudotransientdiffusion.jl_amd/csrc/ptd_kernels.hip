@@ -1812,8 +1812,8 @@ static hipError_t sweep3_run(ptd_ctx* c, const SweepBufs& b) {
     const KP<T> kp = make_kp<T>(c->p);
     // a chunk touches tz + 6 planes from its base (32-bit offsets): one plane fewer than the two-level limit
     const char* env = getenv("PTD_TZ");
-    const int64_t want = env && atoi(env) > 0 ? atoi(env) : t3_chunk_tz(c->p, (int64_t)c->esz);
-    const int tz = (int)std::max<int64_t>(1, std::min<int64_t>(want, max_chunk_tz(c->p, (int64_t)c->esz) - 1));
+    const int64_t want = env && atoi(env) > 0 ? atoi(env) : t3_chunk_tz(c->p, (int64_t)sizeof(T));
+    const int tz = (int)std::max<int64_t>(1, std::min<int64_t>(want, max_chunk_tz(c->p, (int64_t)sizeof(T)) - 1));
     constexpr int R = kRows3;
     const int ntx = (int)((c->p.nx - 2 + 57) / 58);
     const int nty = (int)((c->p.ny - 2 + R - 7) / (R - 6));
@@ -1921,17 +1921,23 @@ hipError_t launch_fused3(ptd_ctx* c) {
     PTD_DISPATCH(fused3_t, c);
 }
 // Three-level passes (3D grids whose chunk keeps its planes under 32-bit offsets) pay where the two-level
-// sweep is bound by its bytes with VALU to spare: the float64 FAST linear and step sweeps. Measured at
-// 511^3, ms per iteration two- / three-level: FAST lin 1.212 / 1.015, step 1.315 / 1.246, nonlinear
-// 1.235 / 1.380; EXACT lin 1.44 / 1.66, step 2.021 / 2.019, nonlinear 2.21 / 2.42; float32 nonlinear
-// 0.653 / 0.907, lin 0.615 / 0.731 (the third level's halo work and registers cost more than its bytes
-// save). PTD_T3=0 / 1 in the environment switches them off / on for every 3D grid.
+// sweep is bound by its bytes with VALU to spare: the FAST linear and step sweeps. Measured at 511^3, ms per
+// iteration two- / three-level (operands two planes ahead, chunk model): float64 FAST lin 1.212 / 0.838,
+// step 1.315 / 1.205, nonlinear 1.235 / 1.284 (level with the model's chunks, not taken); EXACT lin 1.44 /
+// 1.66, step 2.021 / 2.019; float32 lin 0.621 / 0.541, step 0.726 / 0.667, nonlinear 0.653 / 0.779. The
+// float32 passes lose on small grids, where their blocks fill the GPU worse (same context, lin / step:
+// 191^3 0.0243 / 0.0359 and 0.0422 / 0.0467, 255^3 0.085 / 0.085 and 0.101 / 0.107) and win from ~300^3 on
+// (319^3 0.1728 / 0.1412 and 0.1975 / 0.1761, 383^3 0.2754 / 0.2515 and 0.3187 / 0.3126, 767^3 2.054 / 1.859
+// and 2.439 / 2.282, 1023^3 lin 5.09 / 4.25). PTD_T3=0 / 1 in the environment switches them off / on for
+// every 3D grid.
+constexpr int64_t kT3MinCellsF32 = int64_t(24) << 20;  // inner cells, ~294^3
 bool fused3_wanted(const ptd_params& p, int64_t esz) {
     if (!(p.ndim == 3 && p.nx >= 3 && p.ny >= 3 && p.nz >= 3 && max_chunk_tz(p, esz) >= 2)) return false;
     const char* env = getenv("PTD_T3");
     if (env && env[0] == '0') return false;
     if (env && env[0] == '1') return true;
-    return esz == 8 && p.arith == PTD_ARITH_FAST && (p.dcase == PTD_LIN || p.dcase == PTD_STEP);
+    if (p.arith != PTD_ARITH_FAST || (p.dcase != PTD_LIN && p.dcase != PTD_STEP)) return false;
+    return esz == 8 || (p.nx - 2) * (p.ny - 2) * (p.nz - 2) >= kT3MinCellsF32;
 }
 bool fused3_available(const ptd_ctx* c) {
     return fused_available(c) && fused3_wanted(c->p, (int64_t)c->esz);
@@ -2352,6 +2358,39 @@ hipError_t launch_delta_sweep(ptd_ctx* c, bool two, bool check) {
     c->dcur = 1 - d;
     c->pcur = 1 - q;
     return hipSuccess;
+}
+
+// Three float32 correction iterations in one three-level pass (linear and step D; D = H^3 keeps its
+// two-level correction sweep, whose flux is the correction flux around Hc).
+hipError_t launch_delta_sweep3(ptd_ctx* c) {
+    if (c->p.ndim != 3 || (c->p.dcase != PTD_LIN && c->p.dcase != PTD_STEP)) return hipErrorInvalidValue;
+    SweepBufs b;
+    const int d = c->dcur, q = c->pcur;
+    b.Hin = c->dl[d];
+    b.Hout = c->dl[1 - d];
+    b.Hold = c->src;
+    for (int a = 0; a < 3; ++a) {
+        b.qin[a] = c->pq[a][q];
+        b.qout[a] = c->pq[a][1 - q];
+    }
+    hipError_t e;
+    if (c->p.dcase == PTD_STEP) {
+        b.aux = c->Df;
+        e = sweep3_run<float, PTD_STEP, PTD_PERF, kFast>(c, b);
+    } else {
+        b.aux = nullptr;
+        e = sweep3_run<float, PTD_LIN, PTD_PERF, kFast>(c, b);
+    }
+    if (e != hipSuccess) return e;
+    c->dcur = 1 - d;
+    c->pcur = 1 - q;
+    return hipSuccess;
+}
+bool delta_t3(const ptd_ctx* c) {
+    if (c->p.dcase != PTD_LIN && c->p.dcase != PTD_STEP) return false;
+    ptd_params p = c->p;
+    p.arith = PTD_ARITH_FAST;  // the correction sweeps are FAST whatever the float64 passes use
+    return fused3_wanted(p, 4);
 }
 
 // Number of (sum,max) partials the reductions can produce for this context.

@@ -142,3 +142,70 @@ def test_t3_full_size_equals_two_level(script, pkg, monkeypatch):
     assert _relmax(out[0][0], H0) > 1e-9  # the sweeps moved the field
     assert np.array_equal(out[0][0], out[1][0]), f"H: rel {_relmax(out[0][0], out[1][0]):.2e}"
     assert np.array_equal(out[0][1], out[1][1])
+
+
+@pytest.mark.parametrize("script", ["diff_3D_lin_multixpu", "diff_3D_linstep_multixpu"])
+def test_t3_float32_default_rule_equals_two_level(script, pkg, monkeypatch):
+    """float32 FAST linear / step solves take three-level passes by default from ~294^3 inner cells on: at 319^3
+    (the float32 chunk model's depth) 9 iterations give the bits of the two-level sweeps."""
+    n, niter = 319, 9
+    print(f"[gpu] t3 float32 default rule {script} {n}^3", flush=True)
+    monkeypatch.delenv("PTD_TZ", raising=False)
+    spec = pkg.SCRIPTS[script]
+    params = pkg.make_params(spec, n, n, n, arith=1, precision=pkg.FP32)
+    H0 = pkg.initial_condition(spec, n, n, n, kind="config")
+    D = pkg.step_field(spec, n, n, n, kind="half_x") if spec["dcase"] == pkg.STEP else None
+    out = []
+    for t3 in (None, "0"):
+        if t3 is None:
+            monkeypatch.delenv("PTD_T3", raising=False)
+        else:
+            monkeypatch.setenv("PTD_T3", t3)
+        with pkg.Context(params) as ctx:
+            ctx.set("H", H0)
+            ctx.set("Hold", H0)
+            if D is not None:
+                ctx.set("D", D)
+            ctx.iterate(niter)
+            out.append((ctx.get("H"), ctx.get("qHx")))
+    assert np.all(np.isfinite(out[0][0]))
+    assert _relmax(out[0][0], H0) > 1e-6  # the sweeps moved the field
+    assert np.array_equal(out[0][0], out[1][0]), f"H: rel {_relmax(out[0][0], out[1][0]):.2e}"
+    assert np.array_equal(out[0][1], out[1][1])
+
+
+def _mixed_steps(pkg, script, shape, steps):
+    spec = pkg.SCRIPTS[script]
+    params = pkg.make_params(spec, *shape, precision=pkg.MIXED, arith=1)
+    H0 = pkg.initial_condition(spec, *shape, kind="config")
+    D = pkg.step_field(spec, *shape, kind="half_x") if spec["dcase"] == pkg.STEP else None
+    out = []
+    with pkg.Context(params) as ctx:
+        ctx.set("H", H0)
+        ctx.set("Hold", H0)
+        if D is not None:
+            ctx.set("D", D)
+        for _ in range(steps):
+            n, rms, mx = ctx.pt_solve()
+            out.append((int(n), int(ctx.outer_iterations), float(rms), float(mx)))
+            ctx.assign()
+        return out, ctx.get("H")
+
+
+@pytest.mark.parametrize("shape,steps", [((45, 38, 29), 3), ((131, 19, 47), 2), ((319, 319, 319), 1)],
+                         ids=["45x38x29", "131x19x47", "319^3"])
+@pytest.mark.parametrize("script", ["diff_3D_lin_multixpu", "diff_3D_linstep_multixpu"])
+def test_t3_mixed_correction_passes_equal_two_level(script, shape, steps, pkg, monkeypatch):
+    """The mixed solve's float32 correction sweeps for linear / step D as three-level passes (forced on the
+    small grids, the default rule at 319^3) give the float32 sweep counts, correction counts, residuals and
+    final H of the two-level correction sweeps bit for bit."""
+    print(f"[gpu] t3 mixed correction passes {script} {shape}", flush=True)
+    if shape[0] < 300:
+        monkeypatch.setenv("PTD_T3", "1")
+    else:
+        monkeypatch.delenv("PTD_T3", raising=False)
+    got, Hg = _mixed_steps(pkg, script, shape, steps)
+    monkeypatch.setenv("PTD_T3", "0")
+    ref, Hr = _mixed_steps(pkg, script, shape, steps)
+    assert got == ref
+    assert np.array_equal(Hg, Hr)
