@@ -1784,17 +1784,17 @@ static hipError_t sweep2d_lds_run(ptd_ctx* c, const SweepBufs& b, int niter, boo
 }
 
 // z chunk depth of the three-level passes. A block marches tz + 4 planes (level 1 runs two planes beyond
-// each end of its chunk), the float64 passes run one 16-wave block per CU (float32 two), and the blocks go
-// in rounds: the depth of least (rounds) x (tz + 6) up to the 32-bit offset limit of a chunk (tz + 6
-// planes). Measured at 511^3 (float64 FAST, ms per iteration for tz 24 / 32 / 64 / 85 / 102 / 128 / 170):
-// linear 0.883 / 0.842 / 0.828 / 0.820 / 0.803 / 0.864 / 0.867, step 1.310 / 1.199 / 1.149 / 1.127 /
+// each end of its chunk), `slots` blocks run at once (the kernel's 16-wave blocks per CU times the CUs:
+// one per CU for the float64 passes and the float32 step pass, two for the float32 linear one), and the
+// blocks go in rounds: the depth of least (rounds) x (tz + 6) up to the 32-bit offset limit of a chunk
+// (tz + 6 planes). Measured at 511^3 (float64 FAST, ms per iteration for tz 24 / 32 / 64 / 85 / 102 / 128 /
+// 170): linear 0.883 / 0.842 / 0.828 / 0.820 / 0.803 / 0.864 / 0.867, step 1.310 / 1.199 / 1.149 / 1.127 /
 // 1.106 / 1.169 / 1.183 (the model's choice: 102, five chunks of the 509 inner planes).
-int t3_chunk_tz(const ptd_params& p, int64_t esz) {
+int t3_chunk_tz(const ptd_params& p, int64_t esz, int64_t slots) {
     const int64_t mz = p.nz - 2;
     // (deeper than 512 planes a chunk saves nothing measurable; the bound keeps the search short per launch)
     const int64_t lim = std::max<int64_t>(1, std::min<int64_t>(std::min<int64_t>(mz, 512), max_chunk_tz(p, esz) - 1));
     const int64_t tiles = (p.nx - 2 + 57) / 58 * ((p.ny - 2 + kRows3 - 7) / (kRows3 - 6));
-    const int64_t slots = esz == 8 ? 256 : 512;
     int64_t best = lim, best_cost = -1;
     for (int64_t d = std::min<int64_t>(4, lim); d <= lim; ++d) {  // at least 4 planes (or the whole grid)
         const int64_t cost = (tiles * ((mz + d - 1) / d) + slots - 1) / slots * (d + 6);
@@ -1806,15 +1806,32 @@ int t3_chunk_tz(const ptd_params& p, int64_t esz) {
     return (int)best;
 }
 
+// Blocks of `kernel` (block size `threads`) that run at once on the device: its occupancy per CU (registers,
+// LDS) times the CUs; at least one per CU.
+static int64_t resident_blocks(const void* kernel, int threads, int device) {
+    int per_cu = 0, ncu = 0;
+    if (hipOccupancyMaxActiveBlocksPerMultiprocessor(&per_cu, kernel, threads, 0) != hipSuccess || per_cu < 1) {
+        (void)hipGetLastError();
+        per_cu = 1;
+    }
+    if (hipDeviceGetAttribute(&ncu, hipDeviceAttributeMultiprocessorCount, device) != hipSuccess || ncu < 1) {
+        (void)hipGetLastError();
+        ncu = 256;
+    }
+    return (int64_t)per_cu * ncu;
+}
+
 // One three-level pass (three iterations, no residual).
 template <typename T, int DC, int FORM, int ARITH>
 static hipError_t sweep3_run(ptd_ctx* c, const SweepBufs& b) {
     const KP<T> kp = make_kp<T>(c->p);
+    constexpr int R = kRows3;
+    static const int64_t slots =
+        resident_blocks(reinterpret_cast<const void*>(&k_fused3d_t3<T, DC, FORM, ARITH, R>), 64 * R, c->device);
     // a chunk touches tz + 6 planes from its base (32-bit offsets): one plane fewer than the two-level limit
     const char* env = getenv("PTD_TZ");
-    const int64_t want = env && atoi(env) > 0 ? atoi(env) : t3_chunk_tz(c->p, (int64_t)sizeof(T));
+    const int64_t want = env && atoi(env) > 0 ? atoi(env) : t3_chunk_tz(c->p, (int64_t)sizeof(T), slots);
     const int tz = (int)std::max<int64_t>(1, std::min<int64_t>(want, max_chunk_tz(c->p, (int64_t)sizeof(T)) - 1));
-    constexpr int R = kRows3;
     const int ntx = (int)((c->p.nx - 2 + 57) / 58);
     const int nty = (int)((c->p.ny - 2 + R - 7) / (R - 6));
     const int nbz = (int)((c->p.nz - 2 + tz - 1) / tz);
