@@ -1941,7 +1941,7 @@ hipError_t launch_fused3(ptd_ctx* c) {
 // sweep is bound by its bytes with VALU to spare: the FAST linear and step sweeps. Measured at 511^3, ms per
 // iteration two- / three-level (operands two planes ahead, chunk model): float64 FAST lin 1.212 / 0.838,
 // step 1.315 / 1.205, nonlinear 1.235 / 1.284 (level with the model's chunks, not taken); EXACT lin 1.44 /
-// 1.66, step 2.021 / 2.019; float32 lin 0.621 / 0.541, step 0.726 / 0.667, nonlinear 0.653 / 0.779. The
+// 1.66, step 2.021 / 2.019; float32 lin 0.621 / 0.541, step 0.719 / 0.645, nonlinear 0.653 / 0.779. The
 // float32 passes lose on small grids, where their blocks fill the GPU worse (same context, lin / step:
 // 191^3 0.0243 / 0.0359 and 0.0422 / 0.0467, 255^3 0.085 / 0.085 and 0.101 / 0.107) and win from ~300^3 on
 // (319^3 0.1728 / 0.1412 and 0.1975 / 0.1761, 383^3 0.2754 / 0.2515 and 0.3187 / 0.3126, 767^3 2.054 / 1.859
