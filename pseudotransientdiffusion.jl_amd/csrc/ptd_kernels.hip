@@ -273,6 +273,37 @@ __device__ __forceinline__ V shfl_up1(V v) { return __shfl_up(v, 1); }
 template <typename V>
 __device__ __forceinline__ V shfl_down1(V v) { return __shfl_down(v, 1); }
 
+
+// x-neighbour exchange by one lane with DPP wave shifts (v_mov_b32_dpp wave_shr:1 / wave_shl:1) instead of
+// ds_bpermute: lane 0 (shr) and lane 63 (shl) keep their own value, as __shfl_up / __shfl_down do. Only at
+// points where the whole wave is active.
+template <typename V>
+__device__ __forceinline__ V dpp_up1(V v) {
+    if constexpr (sizeof(V) == 4) {
+        const int i = __builtin_bit_cast(int, v);
+        return __builtin_bit_cast(V, __builtin_amdgcn_update_dpp(i, i, 0x138, 0xf, 0xf, false));
+    } else {
+        const unsigned long long u = __builtin_bit_cast(unsigned long long, v);
+        const int lo = (int)(unsigned)u, hi = (int)(unsigned)(u >> 32);
+        const unsigned a = (unsigned)__builtin_amdgcn_update_dpp(lo, lo, 0x138, 0xf, 0xf, false);
+        const unsigned b = (unsigned)__builtin_amdgcn_update_dpp(hi, hi, 0x138, 0xf, 0xf, false);
+        return __builtin_bit_cast(V, ((unsigned long long)b << 32) | a);
+    }
+}
+template <typename V>
+__device__ __forceinline__ V dpp_down1(V v) {
+    if constexpr (sizeof(V) == 4) {
+        const int i = __builtin_bit_cast(int, v);
+        return __builtin_bit_cast(V, __builtin_amdgcn_update_dpp(i, i, 0x130, 0xf, 0xf, false));
+    } else {
+        const unsigned long long u = __builtin_bit_cast(unsigned long long, v);
+        const int lo = (int)(unsigned)u, hi = (int)(unsigned)(u >> 32);
+        const unsigned a = (unsigned)__builtin_amdgcn_update_dpp(lo, lo, 0x130, 0xf, 0xf, false);
+        const unsigned b = (unsigned)__builtin_amdgcn_update_dpp(hi, hi, 0x130, 0xf, 0xf, false);
+        return __builtin_bit_cast(V, ((unsigned long long)b << 32) | a);
+    }
+}
+
 // Operands of one plane p of the march (loaded two planes ahead).
 template <typename T, typename Ta>
 struct PlaneOps {
@@ -1070,16 +1101,16 @@ __device__ __forceinline__ void fused3d_t2_tile(const KP<T>& p, const T* __restr
 
         // phase B: level-1 faces of plane k, level-2 x / y faces of plane k-1
         const int rs = row > 0 ? row - 1 : 0, rn = row < R - 1 ? row + 1 : R - 1;
-        const T Hw = __shfl_up(Hc, 1);
+        const T Hw = dpp_up1(Hc);
         const T Hs = sH[rs][lane];
         Ta Dw{}, De{}, Ds{}, Dnn{}, Dpw{}, Dpe{}, Dps{}, Dpn{};
         if constexpr (AUX) {
-            Dw = shfl_up1(Dc);
-            De = shfl_down1(Dc);
+            Dw = dpp_up1(Dc);
+            De = dpp_down1(Dc);
             Ds = sD[rs][lane];
             Dnn = sD[rn][lane];
-            Dpw = shfl_up1(Dm1);
-            Dpe = shfl_down1(Dm1);
+            Dpw = dpp_up1(Dm1);
+            Dpe = dpp_down1(Dm1);
             Dps = sDp[rs][lane];
             Dpn = sDp[rn][lane];
         }
@@ -1097,7 +1128,7 @@ __device__ __forceinline__ void fused3d_t2_tile(const KP<T>& p, const T* __restr
                 q1zU = Ph::template face<false>(p, A.qz, Hc, A.Hn, Dc, A.Dn, p.dz, p._dz, p._dz8, unused);
             }
         }
-        const T H1w = __shfl_up(H1p, 1);
+        const T H1w = dpp_up1(H1p);
         const T H1s = sH1[rs][lane];
         T r2x = 0, r2y = 0, q2x = 0, q2y = 0;
         if constexpr (LAG) {
@@ -1113,7 +1144,7 @@ __device__ __forceinline__ void fused3d_t2_tile(const KP<T>& p, const T* __restr
         __syncthreads();
 
         // phase C: level-1 update of plane k; level-2 z face (k-1 | k) and update of plane k-1
-        const T q1xE = __shfl_down(q1x, 1);
+        const T q1xE = dpp_down1(q1x);
         const T q1yN = sQ1[rn][lane];
         T dmax1 = 0, dmax2 = 0;
         if constexpr (STEP) {
@@ -1142,14 +1173,14 @@ __device__ __forceinline__ void fused3d_t2_tile(const KP<T>& p, const T* __restr
         if (k > k0) {
             const int m = k - 1;
             if (l2all) {
-                const T q2xE = __shfl_down(q2x, 1);
+                const T q2xE = dpp_down1(q2x);
                 const T q2yN = sQ2[rn][lane];
                 const T dv2 = Ph::template div<3>(p, q2xE - q2x, q2yN - q2y, q2zU - q2zL);
                 T H2;
                 if constexpr (LAG) H2 = Ph::cell_lag(p, H1p, Hop, dv2, v1p);
                 else H2 = Ph::cell(p, H1p, Hop, dv2, dmax2, Dm1);
                 if constexpr (CHECK) {
-                    const T r2xE = __shfl_down(r2x, 1);
+                    const T r2xE = dpp_down1(r2x);
                     const T r2yN = sR2[rn][lane];
                     const T dvr = Ph::template div<3>(p, r2xE - r2x, r2yN - r2y, r2z - r2zL);
                     const T rr = u2 ? Ph::resid(p, H2, Hop, dvr) : (T)0;
@@ -1359,20 +1390,20 @@ __device__ __forceinline__ void fused3d_t3_tile(const KP<T>& p, const T* __restr
         const int rs = row > 0 ? row - 1 : 0, rn = row < R - 1 ? row + 1 : R - 1;
         T Dw0{}, De0{}, Ds0{}, Dn0{}, Dw1{}, De1{}, Ds1{}, Dn1{}, Dw2{}, De2{}, Ds2{}, Dn2{};
         if constexpr (STEP) {
-            Dw0 = shfl_up1(Dc);
-            De0 = shfl_down1(Dc);
+            Dw0 = dpp_up1(Dc);
+            De0 = dpp_down1(Dc);
             Ds0 = sD0[rs][lane];
             Dn0 = sD0[rn][lane];
-            Dw1 = shfl_up1(Dm1);
-            De1 = shfl_down1(Dm1);
+            Dw1 = dpp_up1(Dm1);
+            De1 = dpp_down1(Dm1);
             Ds1 = sD1[rs][lane];
             Dn1 = sD1[rn][lane];
-            Dw2 = shfl_up1(Dm2);
-            De2 = shfl_down1(Dm2);
+            Dw2 = dpp_up1(Dm2);
+            De2 = dpp_down1(Dm2);
             Ds2 = sD2[rs][lane];
             Dn2 = sD2[rn][lane];
         }
-        const T Hw = __shfl_up(Hc, 1);
+        const T Hw = dpp_up1(Hc);
         const T Hs = sH[rs][lane];
         T q1x = 0, q1y = 0, q1zU = 0;
         if (l1y) q1y = Ph::template face<false>(p, A.qy, Hs, Hc, Ds0, Dc, p.dy, p._dy, p._dy8, unused);
@@ -1380,12 +1411,12 @@ __device__ __forceinline__ void fused3d_t3_tile(const KP<T>& p, const T* __restr
             q1x = Ph::template face<false>(p, A.qx, Hw, Hc, Dw0, Dc, p.dx, p._dx, p._dx8, unused);
             q1zU = Ph::template face<false>(p, A.qz, Hc, A.Hn, Dc, A.Dn, p.dz, p._dz, p._dz8, unused);
         }
-        const T H1w = __shfl_up(H1p, 1);
+        const T H1w = dpp_up1(H1p);
         const T H1s = sH1[rs][lane];
         T q2x = 0, q2y = 0;
         if (l2y) q2y = Ph::template face<false>(p, q1yp, H1s, H1p, Ds1, Dm1, p.dy, p._dy, p._dy8, unused);
         if (l2all) q2x = Ph::template face<false>(p, q1xp, H1w, H1p, Dw1, Dm1, p.dx, p._dx, p._dx8, unused);
-        const T H2w = __shfl_up(H2p, 1);
+        const T H2w = dpp_up1(H2p);
         const T H2s = sH2[rs][lane];
         T q3x = 0, q3y = 0;
         if (l3y) q3y = Ph::template face<false>(p, q2yp, H2s, H2p, Ds2, Dm2, p.dy, p._dy, p._dy8, unused);
@@ -1411,7 +1442,7 @@ __device__ __forceinline__ void fused3d_t3_tile(const KP<T>& p, const T* __restr
         }
         T H1 = Hc;  // boundary cells, boundary planes and rows without a level-1 update keep H
         if (l1all) {
-            const T q1xE = __shfl_down(q1x, 1);
+            const T q1xE = dpp_down1(q1x);
             const T q1yN = sQ1[rn][lane];
             const T dv1 = Ph::template div<3>(p, q1xE - q1x, q1yN - q1y, q1zU - q1zL);
             const T H1c = Ph::cell(p, Hc, A.Ho, dv1, dmax1, Dc);
@@ -1420,7 +1451,7 @@ __device__ __forceinline__ void fused3d_t3_tile(const KP<T>& p, const T* __restr
         T q2zU = 0, H2 = H1p;
         if (l2all) {
             q2zU = Ph::template face<false>(p, q1zL, H1p, H1, Dm1, Dc, p.dz, p._dz, p._dz8, unused);
-            const T q2xE = __shfl_down(q2x, 1);
+            const T q2xE = dpp_down1(q2x);
             const T q2yN = sQ2[rn][lane];
             const T dv2 = Ph::template div<3>(p, q2xE - q2x, q2yN - q2y, q2zU - q2zL);
             const T H2c = Ph::cell(p, H1p, Hop, dv2, dmax2, Dm1);
@@ -1430,7 +1461,7 @@ __device__ __forceinline__ void fused3d_t3_tile(const KP<T>& p, const T* __restr
             const T q3zU = Ph::template face<false>(p, q2zL, H2p, H2, Dm2, Dm1, p.dz, p._dz, p._dz8, unused);
             const int m = k - 2;
             if (m >= k0 && m <= k1 - 1) {
-                const T q3xE = __shfl_down(q3x, 1);
+                const T q3xE = dpp_down1(q3x);
                 const T q3yN = sQ3[rn][lane];
                 const T dv3 = Ph::template div<3>(p, q3xE - q3x, q3yN - q3y, q3zU - q3zL);
                 const T H3 = Ph::cell(p, H2p, Hopp, dv3, dmax3, Dm2);
