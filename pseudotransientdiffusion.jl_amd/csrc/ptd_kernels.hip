@@ -267,16 +267,10 @@ __device__ __forceinline__ void stc(char* __restrict__ base, uint32_t off, T v) 
     *reinterpret_cast<T*>(base + off) = v;
 }
 
-// Lane shuffles by one lane (x neighbours).
-template <typename V>
-__device__ __forceinline__ V shfl_up1(V v) { return __shfl_up(v, 1); }
-template <typename V>
-__device__ __forceinline__ V shfl_down1(V v) { return __shfl_down(v, 1); }
-
-
-// x-neighbour exchange by one lane with DPP wave shifts (v_mov_b32_dpp wave_shr:1 / wave_shl:1) instead of
-// ds_bpermute: lane 0 (shr) and lane 63 (shl) keep their own value, as __shfl_up / __shfl_down do. Only at
-// points where the whole wave is active.
+// x-neighbour exchange by one lane (one wavefront is one row of 64 cells) with DPP wave shifts
+// (v_mov_b32_dpp wave_shr:1 / wave_shl:1) instead of ds_bpermute through the LDS crossbar: lane 0 (up) and
+// lane 63 (down) keep their own value, as __shfl_up / __shfl_down would. Only at points where the whole wave
+// is active (every call sits outside divergent branches).
 template <typename V>
 __device__ __forceinline__ V dpp_up1(V v) {
     if constexpr (sizeof(V) == 4) {
@@ -408,12 +402,12 @@ k_fused3d(const KP<T> p, const T* __restrict__ Hin, T* __restrict__ Hout, const 
         if constexpr (AUX) sD[row][lane] = Dc;
         __syncthreads();
 
-        const T Hw = __shfl_up(Hc, 1);
+        const T Hw = dpp_up1(Hc);
         const T Hs = sH[row > 0 ? row - 1 : 0][lane];
         Ta Dw{}, De{}, Ds{}, Dnn{};
         if constexpr (AUX) {
-            Dw = shfl_up1(Dc);
-            De = shfl_down1(Dc);
+            Dw = dpp_up1(Dc);
+            De = dpp_down1(Dc);
             Ds = sD[row > 0 ? row - 1 : 0][lane];
             Dnn = sD[row < R - 1 ? row + 1 : R - 1][lane];
         }
@@ -425,7 +419,7 @@ k_fused3d(const KP<T> p, const T* __restrict__ Hin, T* __restrict__ Hout, const 
         if constexpr (CHECK) sQ2[row][lane] = q2y;
         __syncthreads();
 
-        const T qxE = __shfl_down(qxn, 1);
+        const T qxE = dpp_down1(qxn);
         const T qyN = sQ[row < R - 1 ? row + 1 : R - 1][lane];
         T dmax = 0;
         if constexpr (STEP) {
@@ -436,7 +430,7 @@ k_fused3d(const KP<T> p, const T* __restrict__ Hin, T* __restrict__ Hout, const 
         const T dv = Ph::template div<3>(p, qxE - qxn, qyN - qyn, qzn - qzL);
         const T Hnew = Ph::cell(p, Hc, A.Ho, dv, dmax, Dc);
         if constexpr (CHECK) {
-            const T q2xE = __shfl_down(q2x, 1);
+            const T q2xE = dpp_down1(q2x);
             const T q2yN = sQ2[row < R - 1 ? row + 1 : R - 1][lane];
             const T dv2 = Ph::template div<3>(p, q2xE - q2x, q2yN - q2y, q2z - q2zL);
             const double r = upd ? (double)Ph::resid(p, Hnew, A.Ho, dv2) : 0.0;
@@ -518,12 +512,12 @@ __device__ __forceinline__ void fused2d_tile(const KP<T>& p, const T* __restrict
     if constexpr (STEP) sD[row][lane] = Dc;
     __syncthreads();
 
-    const T Hw = __shfl_up(Hc, 1);
+    const T Hw = dpp_up1(Hc);
     const T Hs = sH[row > 0 ? row - 1 : 0][lane];
     T Dw = 0, De = 0, Ds = 0, Dn = 0;
     if constexpr (STEP) {
-        Dw = __shfl_up(Dc, 1);
-        De = __shfl_down(Dc, 1);
+        Dw = dpp_up1(Dc);
+        De = dpp_down1(Dc);
         Ds = sD[row > 0 ? row - 1 : 0][lane];
         Dn = sD[row < R - 1 ? row + 1 : R - 1][lane];
     }
@@ -534,7 +528,7 @@ __device__ __forceinline__ void fused2d_tile(const KP<T>& p, const T* __restrict
     if constexpr (CHECK) sQ2[row][lane] = q2y;
     __syncthreads();
 
-    const T qxE = __shfl_down(qxn, 1);
+    const T qxE = dpp_down1(qxn);
     const T qyN = sQ[row < R - 1 ? row + 1 : R - 1][lane];
     T dmax = 0;
     if constexpr (STEP) {
@@ -547,7 +541,7 @@ __device__ __forceinline__ void fused2d_tile(const KP<T>& p, const T* __restrict
     if (wx) qxo[oQx] = qxn;
     if (wy) qyo[oQy] = qyn;
     if constexpr (CHECK) {
-        const T q2xE = __shfl_down(q2x, 1);
+        const T q2xE = dpp_down1(q2x);
         const T q2yN = sQ2[row < R - 1 ? row + 1 : R - 1][lane];
         const T dv2 = Ph::template div<2>(p, q2xE - q2x, q2yN - q2y, (T)0);
         const double r = upd ? (double)Ph::resid(p, Hnew, Ho, dv2) : 0.0;
@@ -640,15 +634,15 @@ __device__ __forceinline__ void fused2d_t2_tile(const KP<T>& p, const T* __restr
         if constexpr (STEP) Dnn = D[row_h(jj + 2 < ny ? jj + 2 : ny - 1)];
 
         // level 1, row jj
-        const T Hw = __shfl_up(Hc, 1);
+        const T Hw = dpp_up1(Hc);
         T Dw = 0, De = 0;
         if constexpr (STEP) {
-            Dw = __shfl_up(Dc, 1);
-            De = __shfl_down(Dc, 1);
+            Dw = dpp_up1(Dc);
+            De = dpp_down1(Dc);
         }
         const T q1x = Ph::template face<false>(p, qxo_, Hw, Hc, Dw, Dc, p.dx, p._dx, p._dx8, unused);
         const T q1yU = Ph::template face<false>(p, qyo_, Hc, Hn, Dc, Dn, p.dy, p._dy, p._dy8, unused);
-        const T q1xE = __shfl_down(q1x, 1);
+        const T q1xE = dpp_down1(q1x);
         T dmax1 = 0;
         if constexpr (STEP) {
             dmax1 = fmax(fmax(Dw, De), Dc);
@@ -663,15 +657,15 @@ __device__ __forceinline__ void fused2d_t2_tile(const KP<T>& p, const T* __restr
         const T q2yU = Ph::template face<CHECK>(p, q1yL, H1p, H1, Dp, Dc, p.dy, p._dy, p._dy8, r2yU);
         const int m = jj - 1;
         if (m >= j0) {
-            const T H1w = __shfl_up(H1p, 1);
+            const T H1w = dpp_up1(H1p);
             T Dpw = 0, Dpe = 0;
             if constexpr (STEP) {
-                Dpw = __shfl_up(Dp, 1);
-                Dpe = __shfl_down(Dp, 1);
+                Dpw = dpp_up1(Dp);
+                Dpe = dpp_down1(Dp);
             }
             T r2x = 0;
             const T q2x = Ph::template face<CHECK>(p, q1xp, H1w, H1p, Dpw, Dp, p.dx, p._dx, p._dx8, r2x);
-            const T q2xE = __shfl_down(q2x, 1);
+            const T q2xE = dpp_down1(q2x);
             T dmax2 = 0;
             if constexpr (STEP) {
                 dmax2 = fmax(fmax(Dpw, Dpe), Dp);
@@ -680,7 +674,7 @@ __device__ __forceinline__ void fused2d_t2_tile(const KP<T>& p, const T* __restr
             const T dv2 = Ph::template div<2>(p, q2xE - q2x, q2yU - q2yL, (T)0);
             const T H2 = Ph::cell(p, H1p, Hop, dv2, dmax2, Dp);
             if constexpr (CHECK) {
-                const T r2xE = __shfl_down(r2x, 1);
+                const T r2xE = dpp_down1(r2x);
                 const T dvr = Ph::template div<2>(p, r2xE - r2x, r2yU - r2yL, (T)0);
                 const T rr = l2 ? Ph::resid(p, H2, Hop, dvr) : (T)0;
                 acc += rr * rr;
