@@ -8,12 +8,15 @@ value = T_eff = A_eff / t_it with A_eff = (2*4+2)*nx*ny*nz*8 bytes per iteration
 definition (scripts/diff_3D/diff_3D_nonlin_perf.jl:114-116). K iterations are timed with HIP events on
 the library's stream, bracketed by device synchronisation.
 
-roofline: the dominant kernel is the two-level sweep (one launch = two PT iterations, one pass over the
-fields). achieved = its compulsory HBM bytes per launch, (N + 2r + 6q)*8 (read H over the N cells, Hold
-and write H over the r inner cells, read and write the three flux arrays of q faces each), divided by the
-launch time; frac = achieved / 8 TB/s. traffic = the launch's measured HBM bytes (rocprofv3 FETCH_SIZE +
-WRITE_SIZE, committed profile), traffic_frac = traffic / launch time / 8 TB/s. teff_over_peak = T_eff /
-8 TB/s, which counts the reference's A_eff for each of the launch's two iterations and can exceed 1.
+roofline: the dominant kernel is the sweep the library runs for these iterations (ptd_ctx_iterations_per_launch:
+three-level passes for the 511^3 float64 FAST nonlinear case, one launch = three PT iterations and one pass
+over the fields; two-level passes, two iterations, where it picks those). achieved = its compulsory HBM bytes
+per launch, (N + 2r + 6q)*8 (read H over the N cells, Hold and write H over the r inner cells, read and write
+the three flux arrays of q faces each), divided by the launch time (t_it x iterations per launch; with 200
+steps the last two iterations are a two-level pass); frac = achieved / 8 TB/s. traffic = the launch's
+measured HBM bytes (rocprofv3 FETCH_SIZE + WRITE_SIZE, committed profile), traffic_frac = traffic / launch
+time / 8 TB/s. teff_over_peak = T_eff / 8 TB/s, which counts the reference's A_eff for each of the launch's
+iterations and can exceed 1.
 stream_copy_GBs = a plain 2 GiB device copy timed in the same run (the box's achievable HBM rate), with
 achieved_over_stream and traffic_over_stream beside the fractions of the published peak.
 
@@ -50,7 +53,8 @@ def log(msg):
     print(f"[bench] {msg}", file=sys.stderr, flush=True)
 
 
-TRAFFIC_PROFILE = os.path.join(ROOT, "profiles", "fused3d_t2_nonlin_511_traffic.md")
+TRAFFIC_PROFILES = [os.path.join(ROOT, "profiles", "fused3d_t2_nonlin_511_traffic.md"),
+                    os.path.join(ROOT, "profiles", "fused3d_t3_nonlin_511_traffic.md")]
 
 
 def cpu_model():
@@ -109,19 +113,22 @@ def stream_copy_gbs(repeats=10):
     return 2 * n * 8 / (ms / 1e3) / 1e9
 
 
-def measured_traffic(n, dtype, arith):
-    """HBM bytes per launch of the two-level sweep, from the committed rocprofv3 FETCH_SIZE / WRITE_SIZE
-    summary (line `traffic: n=<n> dtype=<dtype> arith=<arith> read=<B> write=<B>`), or None."""
-    try:
-        lines = open(TRAFFIC_PROFILE).read().splitlines()
-    except OSError:
-        return None
-    for ln in lines:
-        if not ln.startswith("traffic:"):
+def measured_traffic(n, dtype, arith, levels):
+    """HBM bytes per launch of the sweep with `levels` iterations per launch, from the committed rocprofv3
+    FETCH_SIZE / WRITE_SIZE summaries (line `traffic: n=<n> dtype=<dtype> arith=<arith> [levels=<k>]
+    read=<B> write=<B>`, levels 2 when absent), or None."""
+    for path in TRAFFIC_PROFILES:
+        try:
+            lines = open(path).read().splitlines()
+        except OSError:
             continue
-        kv = dict(t.split("=", 1) for t in ln.split()[1:])
-        if kv.get("n") == str(n) and kv.get("dtype") == dtype and kv.get("arith") == arith:
-            return int(kv["read"]) + int(kv["write"])
+        for ln in lines:
+            if not ln.startswith("traffic:"):
+                continue
+            kv = dict(t.split("=", 1) for t in ln.split()[1:])
+            if (kv.get("n") == str(n) and kv.get("dtype") == dtype and kv.get("arith") == arith
+                    and int(kv.get("levels", "2")) == levels):
+                return int(kv["read"]) + int(kv["write"])
     return None
 
 
@@ -475,7 +482,8 @@ def main():
         ctx.set("Hold", H0)
         sync()
         t_up = time.perf_counter() - t_up
-        log(f"warmup {args.warmup} iterations")
+        ipl = ctx.iterations_per_launch()  # the sweep these iterations run (three-, two-level, single)
+        log(f"warmup {args.warmup} iterations ({ipl} per launch)")
         ctx.iterate(args.warmup)
         sync()
         log(f"timing {args.steps} iterations")
@@ -499,11 +507,9 @@ def main():
             ms_exact = None
     t_it = ms / 1e3 / args.steps
     t_eff = a_eff / t_it
-    # the dominant kernel is the two-level sweep: one launch = two PT iterations (an even step count
-    # is all pairs), so its algorithmic bytes per launch are 2*A_eff and its launch time 2*t_it
-    # (achieved = 2*A_eff / (2*t_it) = T_eff)
-    iters_per_launch = 2 if (args.steps % 2 == 0 and os.environ.get("PTD_T2", "1") != "0") else 1
-    traffic = measured_traffic(n, "f64" if prec == pkg.FP64 else "f32", args.arith)
+    # the dominant kernel: one launch = ipl PT iterations, launch time ipl * t_it
+    iters_per_launch = ipl if ipl in (2, 3) else 1
+    traffic = measured_traffic(n, "f64" if prec == pkg.FP64 else "f32", args.arith, iters_per_launch)
     # roofline of the dominant kernel: its compulsory bytes per launch over the launch time
     launch_s = t_it * iters_per_launch
     comp = compulsory_bytes(n, esz)
@@ -526,7 +532,7 @@ def main():
         "config": {"workload": f"3D nonlinear D=H^3 PT iteration (fused flux+update sweep), {n}^3, "
                                f"perf-script expressions, H0=1+2exp(-r^2), H=1 boundaries",
                    "nx": n, "ny": n, "nz": n, "A_eff_GB_per_iter": round(a_eff, 6), "arith": args.arith,
-                   "kernel": "k_fused3d_t2" if iters_per_launch == 2 else "k_fused3d",
+                   "kernel": {3: "k_fused3d_t3", 2: "k_fused3d_t2"}.get(iters_per_launch, "k_fused3d"),
                    "iterations_per_launch": iters_per_launch,
                    "launch_us": round(t_it * iters_per_launch * 1e6, 2)},
         "roofline": {"bound": "hbm", "achieved": round(achieved, 2), "peak": PEAK_HBM_GBS, "unit": "GB/s",

@@ -170,6 +170,11 @@ int32_t ptd_assign(ptd_ctx* ctx);
 int32_t ptd_iterate(ptd_ctx* ctx, int64_t niter);
 /* Same as ptd_iterate, bracketed by HIP events on the context's stream; *ms = elapsed time. */
 int32_t ptd_iterate_timed(ptd_ctx* ctx, int64_t niter, double* ms);
+/* PT iterations per launch of the context's unchecked iterations (ptd_iterate and the runs between checks):
+ * 3 (three-level passes), 2 (two-level passes), 1 (single sweeps) or, for small 2D grids, the LDS-resident
+ * launch's up to 10. No counterpart in the reference: a query for measurement tools (which kernel a run
+ * times), following the library's own choice for these parameters and PTD_T2 / PTD_T3 / PTD_LDS2D. */
+int32_t ptd_ctx_iterations_per_launch(const ptd_ctx* ctx, int32_t* n);
 /* One PT solve at fixed Hold: `while err > tol && iter < itmax` with a check every nout iterations
  * (scripts/diff_3D/diff_3D_nonlin_multixpu.jl:102-115). Returns PTD_ERR_NAN if the error is NaN. */
 int32_t ptd_pt_solve(ptd_ctx* ctx, int64_t* iters, double* err_rms, double* err_max);

@@ -32,7 +32,7 @@ EXPORTS = (
     "ptd_set_field", "ptd_get_field", "ptd_field_count",
     "ptd_compute_iter_params", "ptd_compute_flux", "ptd_compute_update", "ptd_compute_flux_res",
     "ptd_check_res", "ptd_assign",
-    "ptd_iterate", "ptd_iterate_timed", "ptd_pt_solve", "ptd_solve",
+    "ptd_iterate", "ptd_iterate_timed", "ptd_ctx_iterations_per_launch", "ptd_pt_solve", "ptd_solve",
     "ptd_grad_solve", "ptd_pt_solve_noalloc",
     "ptd_batch_create", "ptd_batch_destroy", "ptd_batch_size", "ptd_batch_device_bytes",
     "ptd_batch_set_field", "ptd_batch_get_field", "ptd_batch_solve", "ptd_batch_tiling", "ptd_batch_group_times",
@@ -97,6 +97,7 @@ def load():
         "ptd_assign": (i32, [vp]),
         "ptd_iterate": (i32, [vp, i64]),
         "ptd_iterate_timed": (i32, [vp, i64, dp]),
+        "ptd_ctx_iterations_per_launch": (i32, [vp, ctypes.POINTER(ctypes.c_int32)]),
         "ptd_pt_solve": (i32, [vp, i64p, dp, dp]),
         "ptd_solve": (i32, [vp, i32, i64p, dp]),
         "ptd_grad_solve": (i32, [vp, i32, dp, i64, dp, dp, dp, i64p, i64p]),

@@ -143,6 +143,13 @@ class Context:
         check(self._lib.ptd_iterate_timed(self.handle, int(niter), ctypes.byref(ms)))
         return ms.value
 
+    def iterations_per_launch(self):
+        """PT iterations per launch of this context's unchecked iterations (3 three-level, 2 two-level, 1 single
+        sweeps, up to 10 LDS-resident 2D)."""
+        n = ctypes.c_int32()
+        check(self._lib.ptd_ctx_iterations_per_launch(self.handle, ctypes.byref(n)))
+        return n.value
+
     def pt_solve(self, raise_nan=True):
         it, rms, mx = ctypes.c_int64(), ctypes.c_double(), ctypes.c_double()
         st = self._lib.ptd_pt_solve(self.handle, ctypes.byref(it), ctypes.byref(rms), ctypes.byref(mx))

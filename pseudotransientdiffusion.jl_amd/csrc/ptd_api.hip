@@ -750,6 +750,16 @@ int32_t ptd_iterate_timed(ptd_ctx* ctx, int64_t niter, double* ms) {
     return PTD_OK;
 }
 
+int32_t ptd_ctx_iterations_per_launch(const ptd_ctx* ctx, int32_t* n) {
+    if (!ctx || !n) return ptd::set_error(PTD_ERR_ARG, "ctx or n is NULL");
+    // run_iterations' choice for a long unchecked run
+    const char* t2e = getenv("PTD_T2");
+    const bool t2_env = !(t2e && t2e[0] == '0');
+    if (ptd::use_2d_lds(ctx)) *n = (int32_t)ptd::lds2d_max_iters();
+    else *n = ptd::next_launch_iters(60, false, t2_env && ptd::fused2_available(ctx), t2_env && ptd::fused3_available(ctx));
+    return PTD_OK;
+}
+
 int32_t ptd_pt_solve(ptd_ctx* ctx, int64_t* iters, double* err_rms, double* err_max) {
     int st = device_guard(ctx);
     if (st != PTD_OK) return st;

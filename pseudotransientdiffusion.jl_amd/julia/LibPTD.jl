@@ -10,7 +10,7 @@ module LibPTD
 
 export PtdParams, Context, create, destroy, set_params!, set!, get!, field_count, device_bytes,
        compute_iter_params!, compute_flux!, compute_update!, compute_flux_res!, check_res!, assign!,
-       iterate!, iterate_timed!, pt_solve!, solve!, outer_iterations, grad_solve!,
+       iterate!, iterate_timed!, iterations_per_launch, pt_solve!, solve!, outer_iterations, grad_solve!,
        Batch, batch_create, batch_destroy, batch_set!, batch_get!, batch_solve!, batch_tiling,
        batch_group_times
 
@@ -158,6 +158,13 @@ function iterate_timed!(ctx::Context, niter::Integer)
     ms = Ref(0.0)
     check(ccall((:ptd_iterate_timed, libptd), Int32, (Ptr{Cvoid}, Int64, Ref{Float64}), ctx.handle, niter, ms))
     return ms[]
+end
+
+# PT iterations per launch of the context's unchecked iterations (3 three-level, 2 two-level, 1 single sweeps)
+function iterations_per_launch(ctx::Context)
+    n = Ref(Int32(0))
+    check(ccall((:ptd_ctx_iterations_per_launch, libptd), Int32, (Ptr{Cvoid}, Ref{Int32}), ctx.handle, n))
+    return Int(n[])
 end
 
 function pt_solve!(ctx::Context)

@@ -115,7 +115,7 @@ def test_t3_solve_with_checks_matches_oracle(script, pkg, orc, monkeypatch):
     np.testing.assert_allclose(res.errs, ref["errs"], rtol=1e-12)
 
 
-@pytest.mark.parametrize("script", ["diff_3D_lin_multixpu", "diff_3D_linstep_multixpu"])
+@pytest.mark.parametrize("script", ["diff_3D_lin_multixpu", "diff_3D_linstep_multixpu", "diff_3D_nonlin_perf"])
 def test_t3_full_size_equals_two_level(script, pkg, monkeypatch):
     """At 511^3 the three-level passes take the depth of their own chunk model (102 planes: five chunks of the
     509 inner planes, offsets up to 108 planes from a chunk's base): 9 FAST iterations (three passes) give the
@@ -209,3 +209,51 @@ def test_t3_mixed_correction_passes_equal_two_level(script, shape, steps, pkg, m
     ref, Hr = _mixed_steps(pkg, script, shape, steps)
     assert got == ref
     assert np.array_equal(Hg, Hr)
+
+
+@pytest.mark.parametrize("script", ["diff_3D_nonlin_multixpu", "diff_3D_nonlin_perf"])
+def test_t3_nonlinear_model_choice_equals_two_level(script, pkg, monkeypatch):
+    """float64 FAST D = H^3 takes three-level passes where the model of rounds of blocks favours them (112^3
+    is such a grid): a 2-step solve gives the iteration counts, errors and bits of the two-level sweeps."""
+    n, nt = 112, 2
+    print(f"[gpu] t3 nonlinear model choice {script} {n}^3", flush=True)
+    monkeypatch.delenv("PTD_TZ", raising=False)
+    spec = pkg.SCRIPTS[script]
+    params = pkg.make_params(spec, n, n, n, arith=1)
+    H0 = pkg.initial_condition(spec, n, n, n, kind="config")
+    out = []
+    for t3 in (None, "0"):
+        if t3 is None:
+            monkeypatch.delenv("PTD_T3", raising=False)
+        else:
+            monkeypatch.setenv("PTD_T3", t3)
+        with pkg.Context(params) as ctx:
+            ctx.set("H", H0)
+            ctx.set("Hold", H0)
+            it, er = ctx.solve(nt)
+            out.append((list(it), list(er), ctx.get("H")))
+    assert out[0][0] == out[1][0] and out[0][1] == out[1][1]
+    assert np.array_equal(out[0][2], out[1][2])
+
+
+def test_iterations_per_launch_follows_the_rules(pkg, monkeypatch):
+    """ptd_ctx_iterations_per_launch reports the sweep a context's unchecked iterations run: three-level passes
+    for float64 FAST linear / step D and, where the model of rounds favours them, D = H^3 (511^3 yes, 255^3
+    no); float32 D = H^3 and EXACT two-level; PTD_T3=0 two-level; small 2D grids LDS-resident (10)."""
+    monkeypatch.delenv("PTD_T3", raising=False)
+    cases = [("diff_3D_nonlin_perf", (511, 511, 511), pkg.FP64, 1, 3),
+             ("diff_3D_nonlin_perf", (255, 255, 255), pkg.FP64, 1, 2),
+             ("diff_3D_nonlin_perf", (112, 112, 112), pkg.FP64, 1, 3),
+             ("diff_3D_nonlin_perf", (511, 511, 511), pkg.FP32, 1, 2),
+             ("diff_3D_nonlin_perf", (511, 511, 511), pkg.FP64, 0, 2),
+             ("diff_3D_lin_multixpu", (95, 95, 95), pkg.FP64, 1, 3),
+             ("diff_3D_linstep_multixpu", (319, 319, 319), pkg.FP32, 1, 3),
+             ("diff_2D_lin", (512, 512), pkg.FP64, 0, 10)]
+    for script, shape, prec, arith, want in cases:
+        params = pkg.make_params(pkg.SCRIPTS[script], *shape, precision=prec, arith=arith)
+        with pkg.Context(params) as ctx:
+            assert ctx.iterations_per_launch() == want, (script, shape, prec, arith)
+    monkeypatch.setenv("PTD_T3", "0")
+    params = pkg.make_params(pkg.SCRIPTS["diff_3D_nonlin_perf"], 511, 511, 511, arith=1)
+    with pkg.Context(params) as ctx:
+        assert ctx.iterations_per_launch() == 2
