@@ -1358,6 +1358,13 @@ __device__ __forceinline__ void fused3d_t3_tile(const KP<T>& p, const T* __restr
     // ms per iteration: linear 0.998 -> 0.836, step 1.236 -> 1.200, nonlinear 1.370 -> 1.284; three planes
     // ahead were slower again). The float64 EXACT step pass keeps one plane: it would spill.
     constexpr bool kAhead2 = !(sizeof(T) == 8 && DC == PTD_STEP && ARITH == kExact);
+    // The float64 FAST linear / nonlinear passes alternate two operand sets (even steps from ka on A, odd
+    // ones on B, each reloaded two planes ahead right after its step): a copy B -> A at the end of every
+    // step would wait for the loads just issued into B, leaving them one plane step to return. They have
+    // the register room (one 16-wave block per CU either way: 90 -> 96 VGPRs nonlinear; nonlinear at 511^3
+    // 1.171-1.178 -> 1.124-1.137 ms per iteration, 1023^3 9.42-9.54 -> 9.07-9.08); float32 would drop to one
+    // block per CU and float64 step D would spill.
+    constexpr bool kAlt = sizeof(T) == 8 && ARITH == kFast && DC != PTD_STEP;
     PlaneOps2<T, T> A = load_plane(ka);
     PlaneOps2<T, T> B;
     if constexpr (kAhead2) B = load_plane(ka + 1 <= nz - 1 ? ka + 1 : nz - 1);
@@ -1368,7 +1375,7 @@ __device__ __forceinline__ void fused3d_t3_tile(const KP<T>& p, const T* __restr
     T q2xp = 0, q2yp = 0;        // level-2 fluxes on the lower x / y faces at plane k-2
     T q2zL = 0, q3zL = 0;        // level-2 flux on face (k-2 | k-1), level-3 flux on face (k-3 | k-2)
 
-    for (int k = ka; k <= k1 + 1; ++k) {
+    auto step = [&](const int k, PlaneOps2<T, T>& A) {
         // phase A: planes into LDS
         sH[row][lane] = Hc;
         sH1[row][lane] = H1p;
@@ -1489,11 +1496,23 @@ __device__ __forceinline__ void fused3d_t3_tile(const KP<T>& p, const T* __restr
             Dc = A.Dn;
         }
         Hc = A.Hn;
-        if constexpr (kAhead2) {
-            A = B;
-            B = load_plane(k + 2 <= nz - 1 ? k + 2 : nz - 1);
-        } else {
-            A = load_plane(k + 1 <= nz - 1 ? k + 1 : nz - 1);
+        if constexpr (kAlt) A = load_plane(k + 2 <= nz - 1 ? k + 2 : nz - 1);
+    };
+    if constexpr (kAlt) {
+        for (int k = ka; k <= k1 + 1; k += 2) {
+            step(k, A);
+            if (k + 1 > k1 + 1) break;
+            step(k + 1, B);
+        }
+    } else {
+        for (int k = ka; k <= k1 + 1; ++k) {
+            step(k, A);
+            if constexpr (kAhead2) {
+                A = B;
+                B = load_plane(k + 2 <= nz - 1 ? k + 2 : nz - 1);
+            } else {
+                A = load_plane(k + 1 <= nz - 1 ? k + 1 : nz - 1);
+            }
         }
     }
 }
