@@ -1358,16 +1358,19 @@ __device__ __forceinline__ void fused3d_t3_tile(const KP<T>& p, const T* __restr
     // ms per iteration: linear 0.998 -> 0.836, step 1.236 -> 1.200, nonlinear 1.370 -> 1.284; three planes
     // ahead were slower again). The float64 EXACT step pass keeps one plane: it would spill.
     constexpr bool kAhead2 = !(sizeof(T) == 8 && DC == PTD_STEP && ARITH == kExact);
-    // The float64 FAST linear / nonlinear passes alternate two operand sets (even steps from ka on A, odd
-    // ones on B, each reloaded two planes ahead right after its step): a copy B -> A at the end of every
-    // step would wait for the loads just issued into B, leaving them one plane step to return. They have
-    // the register room (one 16-wave block per CU either way: 90 -> 96 VGPRs nonlinear; nonlinear at 511^3
-    // 1.171-1.178 -> 1.124-1.137 ms per iteration, 1023^3 9.42-9.54 -> 9.07-9.08); float32 would drop to one
-    // block per CU and float64 step D would spill.
+    // The float64 FAST linear / nonlinear passes rotate three operand sets (steps ka, ka+3, ... on A, the
+    // next ones on B, then C; each set is reloaded three planes ahead right after its step): a copy B -> A
+    // at the end of every step would wait for the loads just issued into B, leaving them one plane step to
+    // return. They have the register room (one 16-wave block per CU either way: 90 -> 109 VGPRs
+    // nonlinear). Nonlinear at 511^3, ms per iteration: copy 1.171-1.178, two alternating sets
+    // 1.124-1.137; 1023^3 9.42-9.54 -> 9.07-9.08; three sets a further 0.8% (511^3) and 1.3% (1023^3).
+    // float32 would drop to one block per CU and float64 step D would spill.
     constexpr bool kAlt = sizeof(T) == 8 && ARITH == kFast && DC != PTD_STEP;
     PlaneOps2<T, T> A = load_plane(ka);
     PlaneOps2<T, T> B;
     if constexpr (kAhead2) B = load_plane(ka + 1 <= nz - 1 ? ka + 1 : nz - 1);
+    PlaneOps2<T, T> C;
+    if constexpr (kAlt) C = load_plane(ka + 2 <= nz - 1 ? ka + 2 : nz - 1);
 
     T H1p = 0, H2p = 0;          // H1 at plane k-1, H2 at plane k-2
     T Hop = 0, Hopp = 0;         // Hold at planes k-1, k-2
@@ -1496,13 +1499,15 @@ __device__ __forceinline__ void fused3d_t3_tile(const KP<T>& p, const T* __restr
             Dc = A.Dn;
         }
         Hc = A.Hn;
-        if constexpr (kAlt) A = load_plane(k + 2 <= nz - 1 ? k + 2 : nz - 1);
+        if constexpr (kAlt) A = load_plane(k + 3 <= nz - 1 ? k + 3 : nz - 1);
     };
     if constexpr (kAlt) {
-        for (int k = ka; k <= k1 + 1; k += 2) {
+        for (int k = ka; k <= k1 + 1; k += 3) {
             step(k, A);
             if (k + 1 > k1 + 1) break;
             step(k + 1, B);
+            if (k + 2 > k1 + 1) break;
+            step(k + 2, C);
         }
     } else {
         for (int k = ka; k <= k1 + 1; ++k) {
