@@ -2000,16 +2000,18 @@ constexpr int64_t kT3MinCellsF32 = int64_t(24) << 20;  // inner cells, ~294^3
 // float64 FAST D = H^3: the three-level pass (one 16-wave block per CU) and the two-level sweep (two
 // 12-wave blocks per CU) are close, and which one wins on a grid depends on how their blocks fill the rounds.
 // The model: cost per iteration = (rounds of blocks) x (planes a block marches) / (iterations per launch),
-// each with its own chunk depth. Below ~400^3 inner cells three-level passes are taken where they model
-// below 95% of the two-level sweep (at least ~77^3: smaller grids are launch-bound); from ~400^3 on, where a
-// three-level plane step costs less than the model's equal-cost assumption, below 107%. Measured with the
-// lane shifts (ms per iteration two- / three-level, model ratio): 80^3 0.0128 / 0.0114 (0.917), 112^3 0.0202 /
-// 0.0175 (0.889), 159^3 0.0510 / 0.0429 (0.905), 191^3 0.0846 / 0.0846 (0.958), 223^3 0.1290 / 0.1162 (0.944),
-// 255^3 0.1775 / 0.1987 (1.056), 287^3 0.2413 / 0.2246 (0.870), 351^3 0.4156 / 0.4331 (1.096), 383^3 0.552 /
-// 0.564 (1.125), 447^3 0.867 / 0.829 (0.979), 511^3 1.235 / 1.151 (1.000), 639^3 2.384 / 2.208 (0.966), 767^3
-// 4.175 / 4.034 (1.056), 1023^3 10.18 / 9.24 (0.977); 63^3 0.0086 / 0.0106 (0.833, below the size floor).
+// each with its own chunk depth. A three-level plane step costs less than the model's equal-cost assumption
+// (measured time ratio about 0.9 x the model's). Below ~347^3 inner cells three-level passes are taken where
+// they model below 97% of the two-level sweep (at least ~77^3: smaller grids are launch-bound); from ~347^3
+// on, below 113%. Measured with the operand rotation, same context (ms per iteration two- / three-level,
+// model ratio): 159^3 0.0502 / 0.0393 (0.905), 223^3 0.1288 / 0.1089 (0.944), 255^3 0.1719 / 0.1806 (1.056),
+// 287^3 0.2331 / 0.2049 (0.870), 319^3 0.3368 / 0.2871 (0.956), 351^3 0.4057 / 0.3952 (1.096), 383^3 0.5398 /
+// 0.5275 (1.125), 415^3 0.6513 / 0.6416 (1.067), 447^3 0.8529 / 0.7730 (0.979), 511^3 1.2229 / 1.0957
+// (1.000), 639^3 2.3791 / 2.1000 (0.966). Before the rotation (lane shifts): 80^3 0.0128 / 0.0114 (0.917),
+// 112^3 0.0202 / 0.0175 (0.889), 191^3 0.0846 / 0.0846 (0.958), 767^3 4.175 / 4.034 (1.056), 1023^3 10.18 /
+// 9.24 (0.977); 63^3 0.0086 / 0.0106 (0.833, below the size floor).
 constexpr int64_t kT3MinCellsNonlin = int64_t(448) << 10;  // inner cells, ~77^3
-constexpr int64_t kT3LargeCellsNonlin = int64_t(61) << 20;  // inner cells, ~400^3
+constexpr int64_t kT3LargeCellsNonlin = int64_t(40) << 20;  // inner cells, ~347^3
 static bool nonlin_t3_model(const ptd_params& p) {
     const int64_t cells = (p.nx - 2) * (p.ny - 2) * (p.nz - 2);
     if (cells < kT3MinCellsNonlin) return false;
@@ -2020,7 +2022,7 @@ static bool nonlin_t3_model(const ptd_params& p) {
     const int64_t mz = p.nz - 2;
     const int64_t tiles3 = (p.nx - 2 + 57) / 58 * ((p.ny - 2 + kRows3 - 7) / (kRows3 - 6));
     const double c3 = (double)((tiles3 * ((mz + tz3 - 1) / tz3) + slots3 - 1) / slots3 * (tz3 + 6)) / 3.0;
-    return c3 < (cells >= kT3LargeCellsNonlin ? 1.07 : 0.95) * c2;
+    return c3 < (cells >= kT3LargeCellsNonlin ? 1.13 : 0.97) * c2;
 }
 bool fused3_wanted(const ptd_params& p, int64_t esz) {
     if (!(p.ndim == 3 && p.nx >= 3 && p.ny >= 3 && p.nz >= 3 && max_chunk_tz(p, esz) >= 2)) return false;

@@ -238,11 +238,13 @@ def test_t3_nonlinear_model_choice_equals_two_level(script, pkg, monkeypatch):
 
 def test_iterations_per_launch_follows_the_rules(pkg, monkeypatch):
     """ptd_ctx_iterations_per_launch reports the sweep a context's unchecked iterations run: three-level passes
-    for float64 FAST linear / step D and, where the model of rounds favours them, D = H^3 (511^3 yes, 255^3
-    no); float32 D = H^3 and EXACT two-level; PTD_T3=0 two-level; small 2D grids LDS-resident (10)."""
+    for float64 FAST linear / step D and, where the model of rounds favours them, D = H^3 (511^3, 383^3 and 319^3 yes,
+    255^3 no); float32 D = H^3 and EXACT two-level; PTD_T3=0 two-level; small 2D grids LDS-resident (10)."""
     monkeypatch.delenv("PTD_T3", raising=False)
     cases = [("diff_3D_nonlin_perf", (511, 511, 511), pkg.FP64, 1, 3),
              ("diff_3D_nonlin_perf", (255, 255, 255), pkg.FP64, 1, 2),
+             ("diff_3D_nonlin_perf", (319, 319, 319), pkg.FP64, 1, 3),
+             ("diff_3D_nonlin_perf", (383, 383, 383), pkg.FP64, 1, 3),
              ("diff_3D_nonlin_perf", (112, 112, 112), pkg.FP64, 1, 3),
              ("diff_3D_nonlin_perf", (511, 511, 511), pkg.FP32, 1, 2),
              ("diff_3D_nonlin_perf", (511, 511, 511), pkg.FP64, 0, 2),
