@@ -1358,14 +1358,16 @@ __device__ __forceinline__ void fused3d_t3_tile(const KP<T>& p, const T* __restr
     // ms per iteration: linear 0.998 -> 0.836, step 1.236 -> 1.200, nonlinear 1.370 -> 1.284; three planes
     // ahead were slower again). The float64 EXACT step pass keeps one plane: it would spill.
     constexpr bool kAhead2 = !(sizeof(T) == 8 && DC == PTD_STEP && ARITH == kExact);
-    // The float64 FAST linear / nonlinear passes rotate three operand sets (steps ka, ka+3, ... on A, the
-    // next ones on B, then C; each set is reloaded three planes ahead right after its step): a copy B -> A
-    // at the end of every step would wait for the loads just issued into B, leaving them one plane step to
-    // return. They have the register room (one 16-wave block per CU either way: 90 -> 109 VGPRs
-    // nonlinear). Nonlinear at 511^3, ms per iteration: copy 1.171-1.178, two alternating sets
-    // 1.124-1.137; 1023^3 9.42-9.54 -> 9.07-9.08; three sets a further 0.8% (511^3) and 1.3% (1023^3).
-    // float32 would drop to one block per CU and float64 step D would spill.
-    constexpr bool kAlt = sizeof(T) == 8 && ARITH == kFast && DC != PTD_STEP;
+    // The float64 FAST linear / nonlinear passes and the float32 FAST step pass rotate three operand sets
+    // (steps ka, ka+3, ... on A, the next ones on B, then C; each set is reloaded three planes ahead right
+    // after its step): a copy B -> A at the end of every step would wait for the loads just issued into B,
+    // leaving them one plane step to return. They have the register room (one 16-wave block per CU either
+    // way: float64 nonlinear 90 -> 109 VGPRs, float32 step 84 -> 98). Float64 nonlinear at 511^3, ms per
+    // iteration: copy 1.171-1.178, two alternating sets 1.124-1.137; 1023^3 9.42-9.54 -> 9.07-9.08; three
+    // sets a further 0.8% (511^3) and 1.3% (1023^3). Float32 step: 511^3 0.597 -> 0.527, 767^3 2.082 ->
+    // 1.840. The float32 linear pass would drop to one block per CU (56 -> 72 VGPRs; 511^3 2-3% slower,
+    // 767^3 10% faster), and the float64 step pass would spill 64 VGPRs.
+    constexpr bool kAlt = ARITH == kFast && (sizeof(T) == 8 ? DC != PTD_STEP : DC == PTD_STEP);
     PlaneOps2<T, T> A = load_plane(ka);
     PlaneOps2<T, T> B;
     if constexpr (kAhead2) B = load_plane(ka + 1 <= nz - 1 ? ka + 1 : nz - 1);
